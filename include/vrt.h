@@ -1,0 +1,319 @@
+/*
+ * vrt.h - C ABI of the MI355X brickmap ray traversal (libvoxelrt_hip.so).
+ *
+ * The world is a two-level brickmap: a sector is 4x4x4 bricks (32^3 voxels),
+ * a brick is 8x8x8 voxels of one byte each (0 = empty, 1..255 = material id).
+ * On the device it is held as structure-of-arrays buffers in HBM:
+ *   sector masks  u64 [nsec]       bit b set = brick b of the sector is allocated
+ *   base slots    u32 [nsec]       slot of the sector's first allocated brick
+ *   occupancy     u64 [slots * 8]  per brick, one mask per 4x4x4 voxel cell
+ *   voxels        u8  [slots * 512]
+ *   palette       u64 [256]
+ * Brick b of a sector lives in slot base + popcount(mask & ((1 << b) - 1)).
+ *
+ * Every entry point returns a vrt_status; on failure vrt_last_error() gives a
+ * thread-local message.  No C++ exception crosses this boundary.  Device work
+ * is asynchronous on the given stream (a hipStream_t, NULL = the null stream);
+ * buffers passed to casts are device pointers owned by the caller.
+ *
+ * Reference interfaces replaced (paths relative to the VoxelRT source tree):
+ *   vrt_world_*            GpuVoxelStorage / FlatVoxelStorage SyncBuffers
+ *                          (src/VoxelRT/GpuRenderer.cpp:17-167,
+ *                           src/VoxelRT/CpuRenderer.cpp:20-84)
+ *   vrt_cast               RayCast / GetStepPos / GetVoxelMaterial
+ *                          (src/VoxelRT/CpuRenderer.cpp:110-224)
+ *   vrt_cast_one_f64       HitResult VoxelMap::RayCast(dvec3, dvec3, uint32_t)
+ *                          (src/VoxelRT/VoxelMap.h:213-214)
+ *   vrt_bounce_generate    diffuse bounce construction in RenderRow
+ *                          (src/VoxelRT/CpuRenderer.cpp:342-392,
+ *                           src/VoxelRT/Shaders/VoxelRender.comp:63-86)
+ */
+#ifndef VRT_H
+#define VRT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VRT_ABI_VERSION 1
+
+/* ---------------------------------------------------------------- errors */
+
+typedef enum vrt_status {
+    VRT_OK = 0,
+    VRT_ERR_INVALID_ARGUMENT = 1, /* null or short buffer, bad extent, n = 0 where n > 0 is needed */
+    VRT_ERR_CAPACITY = 2,         /* brick slots exhausted ("Could not allocate brick slots") */
+    VRT_ERR_HIP = 3,              /* a HIP runtime call failed */
+    VRT_ERR_NO_DEVICE = 4,        /* no usable gfx950 device */
+    VRT_ERR_STATE = 5             /* call not valid in the object's current state */
+} vrt_status;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* vrt_last_error(void);
+int vrt_abi_version(void);
+
+/* ---------------------------------------------------------------- world */
+
+typedef struct vrt_world vrt_world;
+
+typedef struct vrt_world_desc {
+    uint32_t log2_size_xz;   /* voxel extent along x and z is 1 << log2_size_xz, 5..14 */
+    uint32_t log2_size_y;    /* voxel extent along y is 1 << log2_size_y, 5..14 */
+    uint32_t initial_slots;  /* device brick slots reserved up front (0 = grow on demand) */
+    uint32_t max_slots;      /* slot arena capacity (0 = 64 per sector, i.e. never exhausted) */
+} vrt_world_desc;
+
+/* Shape of a created world (all derived from the descriptor). */
+typedef struct vrt_world_info {
+    uint32_t size_xz, size_y;      /* voxel extents */
+    uint32_t sectors_xz, sectors_y;/* sector grid (32 voxels per sector edge) */
+    uint32_t num_sectors;
+    uint32_t num_bricks;           /* bricks currently allocated */
+    uint32_t slot_capacity;        /* device slots currently backed by memory */
+    uint64_t device_bytes;         /* HBM held by the world buffers */
+} vrt_world_info;
+
+int vrt_world_create(const vrt_world_desc* desc, void* stream, vrt_world** out);
+int vrt_world_destroy(vrt_world* world);
+int vrt_world_get_info(const vrt_world* world, vrt_world_info* out);
+
+/*
+ * Replace the contents of a set of sectors (the DirtyLocs protocol of
+ * GpuRenderer.cpp:45-167).  For each i < nsec:
+ *   sector_idx[i]  linear sector index x | z << sx | y << 2sx (sx = log2 sectors_xz)
+ *   alloc_mask[i]  bricks present in the sector after the update (0 = sector emptied)
+ * bricks holds, sector after sector in the order given, popcount(alloc_mask[i])
+ * bricks of 512 bytes each in increasing brick index; brick voxel order is
+ * x | z << 3 | y << 6.  Slots are (re)allocated on the host, brick bytes are
+ * copied to HBM and the occupancy masks of every uploaded brick are rebuilt on
+ * the device.  Host pointers; the call returns once the host buffers may be
+ * reused (uploads are staged), the device work is ordered on `stream`.
+ * Bricks that are entirely empty may be listed; they stay allocated.
+ */
+int vrt_world_sync(vrt_world* world, uint32_t nsec, const uint32_t* sector_idx,
+                   const uint64_t* alloc_mask, const uint8_t* bricks, void* stream);
+
+/* Palette entries (Material::GetEncoded layout, VoxelMap.h:27-41). Host pointer, 256 entries. */
+int vrt_world_set_palette(vrt_world* world, const uint64_t* palette, void* stream);
+
+/* Device views of the world buffers (read-only use by callers). */
+typedef struct vrt_world_buffers {
+    const uint64_t* sector_mask;  /* [num_sectors] */
+    const uint32_t* base_slot;    /* [num_sectors] */
+    const uint64_t* occupancy;    /* [slot_capacity * 8] */
+    const uint8_t* voxels;        /* [slot_capacity * 512] */
+    const uint64_t* palette;      /* [256] */
+} vrt_world_buffers;
+int vrt_world_get_buffers(const vrt_world* world, vrt_world_buffers* out);
+
+/* ---------------------------------------------------------------- rays and hits */
+
+/* n rays in structure-of-arrays form; origins are relative to world_origin. */
+typedef struct vrt_rays {
+    const float* ox; const float* oy; const float* oz;
+    const float* dx; const float* dy; const float* dz;
+    int32_t world_origin[3];
+    uint32_t n;
+} vrt_rays;
+
+/*
+ * Hit record, 16 bytes per ray.
+ *   x, y, z  absolute coordinates of the hit voxel (0 on a miss)
+ *   info     bits 0-7   material id of the hit voxel (0 = miss)
+ *            bits 8-13  face normal, 2 bits per axis (x, y, z) in two's
+ *                       complement: 0 -> 0, 1 -> +1, 3 -> -1
+ *            bits 16-31 traversal iterations (mask lookups) spent on the ray
+ */
+typedef struct vrt_hit {
+    int32_t x, y, z;
+    uint32_t info;
+} vrt_hit;
+
+typedef struct vrt_hits {
+    float* t;        /* [n] distance along d to the entry face; -1 on a miss */
+    vrt_hit* rec;    /* [n] */
+    float* pos;      /* optional [3 * n] SoA hit position relative to world_origin (NULL = skip) */
+} vrt_hits;
+
+/* Totals over a launch, accumulated into device memory when requested. */
+typedef struct vrt_cast_counters {
+    unsigned long long rays;      /* rays traced */
+    unsigned long long iters;     /* sector-mask lookups (GetStepPos evaluations) */
+    unsigned long long descents;  /* lookups that descended into an allocated brick */
+    unsigned long long hits;      /* rays that hit a voxel */
+} vrt_cast_counters;
+
+/*
+ * Plain per-ray cast: the drop-in for RayCast (CpuRenderer.cpp:172-224),
+ * FP32, un-biased t, +0.001 step bias, 2-level mask LOD.  max_iters is the
+ * iteration cap (the reference uses 128), 1..65535.
+ */
+int vrt_cast(const vrt_world* world, const vrt_rays* rays, uint32_t max_iters,
+             vrt_hits* hits, void* stream);
+
+/* Kernel variants of the same cast (results are identical for every variant). */
+typedef enum vrt_cast_schedule {
+    VRT_SCHED_AUTO = 0,        /* pick from the autotuner cache, else the default */
+    VRT_SCHED_GRID = 1,        /* one ray per lane, one lane per ray, grid covers n */
+    VRT_SCHED_PERSISTENT = 2   /* persistent waves pull ray chunks from a queue and refill idle lanes */
+} vrt_cast_schedule;
+
+typedef struct vrt_cast_opts {
+    uint32_t schedule;          /* vrt_cast_schedule */
+    uint32_t rays_per_lane;     /* 1 or 2 independent rays interleaved per lane (persistent only) */
+    uint32_t chunk;             /* rays per queue grab, multiple of 64 (persistent only, 0 = default) */
+    uint32_t waves_per_cu;      /* persistent occupancy target (0 = default) */
+    vrt_cast_counters* counters;/* optional device pointer, accumulated (not cleared) */
+} vrt_cast_opts;
+
+int vrt_cast_ex(const vrt_world* world, const vrt_rays* rays, uint32_t max_iters,
+                vrt_hits* hits, const vrt_cast_opts* opts, void* stream);
+
+/* Bytes of device scratch a persistent cast needs (queue counters). */
+size_t vrt_cast_scratch_bytes(void);
+
+/*
+ * Scalar FP64 cast of one ray on the device, the semantics of VoxelMap::RayCast
+ * (VoxelMap.cpp:125-170): bias +0.0001, fixed 32/8/1 levels, normal from
+ * tmin >= sideDist, distance = biased tmin.  Synchronous; host in/out.
+ * Returns the hit in *out (distance <= 0 means a miss).
+ */
+typedef struct vrt_hit_f64 {
+    double distance;           /* -1 on a miss */
+    float normal[3];
+    float uv[2];
+    int32_t voxel[3];
+} vrt_hit_f64;
+int vrt_cast_one_f64(const vrt_world* world, const double origin[3], const double dir[3],
+                     uint32_t max_iters, vrt_hit_f64* out);
+
+/* ---------------------------------------------------------------- bounce rays */
+
+/*
+ * Diffuse bounce rays for the hits of one primary set (CpuRenderer.cpp:389-392,
+ * VoxelRender.comp:63-71): for every hit i, origin = pos_i + N_i * 0.01,
+ * direction = normalize(N_i + s), s uniform on the unit sphere from the PCG
+ * stream random_init(uvec3(i % width, i / width, seed)) (RandomGen.glsl:6-20);
+ * a degenerate N + s = 0 falls back to N.  Bounce rays are written compacted
+ * in increasing i; src[k] is the primary index of bounce k; *count (device)
+ * receives the number written.  hits must have pos.  scratch: device buffer of
+ * vrt_bounce_scratch_bytes(n) bytes.
+ */
+typedef struct vrt_ray_buffers {
+    float* ox; float* oy; float* oz;
+    float* dx; float* dy; float* dz;
+} vrt_ray_buffers;
+
+size_t vrt_bounce_scratch_bytes(uint32_t n);
+int vrt_bounce_generate(const vrt_rays* primary, const vrt_hits* hits, uint32_t width,
+                        uint32_t seed, vrt_ray_buffers* out, uint32_t* src, uint32_t* count,
+                        void* scratch, void* stream);
+
+/* ---------------------------------------------------------------- cameras */
+
+/*
+ * Pinhole camera.  Ray for pixel (px, py), before tiling:
+ *   u = (px + 0.5) * inv_w * 2 - 1,  v = 1 - (py + 0.5) * inv_h * 2
+ *   d = normalize(forward + u * right + v * up)   (right/up pre-scaled by the FOV)
+ * a zero component of d is replaced by 2^-24; origin = eye (relative to
+ * world_origin).  Rays are numbered tile by tile: tiles of tile_w x tile_h
+ * pixels in row-major tile order, pixels row-major inside a tile.
+ */
+typedef struct vrt_camera {
+    int32_t world_origin[3];
+    float eye[3];
+    float forward[3];
+    float right[3];
+    float up[3];
+    uint32_t width, height;
+    uint32_t tile_w, tile_h;   /* 1x1 = plain row-major */
+} vrt_camera;
+
+/* Generate the primary rays of a camera into device buffers (n = width*height). */
+int vrt_camera_rays(const vrt_camera* cam, vrt_ray_buffers* out, void* stream);
+
+/* ---------------------------------------------------------------- batched */
+
+typedef enum vrt_segment_kind {
+    VRT_SEG_CAMERA = 0,  /* rays generated in-kernel from a vrt_camera */
+    VRT_SEG_RAYS = 1     /* rays read from buffers (e.g. bounce sets) */
+} vrt_segment_kind;
+
+/*
+ * One segment of a batch.  Hits of segment s are written at hit_offset .. +count.
+ * For VRT_SEG_RAYS, rays are read from the batch's ray buffers at ray_offset,
+ * with the segment's own world origin.  count_ptr, when non-NULL, is a device
+ * u32 that overrides count at run time (count is then the capacity).
+ */
+typedef struct vrt_segment {
+    uint32_t kind;
+    uint32_t camera;          /* index into the camera table (VRT_SEG_CAMERA) */
+    uint32_t ray_offset;      /* first ray in the ray buffers (VRT_SEG_RAYS) */
+    uint32_t hit_offset;      /* first hit in the output */
+    uint32_t count;           /* rays in the segment (capacity when count_ptr is set) */
+    int32_t world_origin[3];  /* VRT_SEG_RAYS */
+    const uint32_t* count_ptr;
+} vrt_segment;
+
+typedef struct vrt_batch {
+    const vrt_camera* cameras; uint32_t ncameras;  /* host */
+    const vrt_segment* segments; uint32_t nsegments; /* host */
+    vrt_rays rays;           /* buffers for VRT_SEG_RAYS segments (world_origin, n unused) */
+    vrt_hits hits;           /* output, indexed by hit_offset */
+    uint32_t max_iters;
+} vrt_batch;
+
+typedef struct vrt_batch_tuning {
+    uint32_t tile_w, tile_h;   /* lane -> pixel tile for camera segments: 8x8 or 16x4 */
+    uint32_t rays_per_lane;    /* 1 or 2 */
+    uint32_t chunk;            /* rays per queue grab, multiple of 64 */
+} vrt_batch_tuning;
+
+typedef struct vrt_batch_plan vrt_batch_plan;
+
+/* Build a plan (device segment/camera tables, queue scratch) for a batch. */
+int vrt_batch_plan_create(const vrt_world* world, const vrt_batch* batch,
+                          const vrt_batch_tuning* tuning, void* stream, vrt_batch_plan** out);
+/* Trace every segment of the plan in one persistent launch. */
+int vrt_batch_launch(vrt_batch_plan* plan, vrt_cast_counters* counters, void* stream);
+/* Capture vrt_batch_launch into a hipGraph (once) and replay it. */
+int vrt_batch_graph_launch(vrt_batch_plan* plan, void* stream);
+int vrt_batch_plan_destroy(vrt_batch_plan* plan);
+
+/*
+ * Time every tuning in the search space on this batch (tile 8x8/16x4, rays per
+ * lane 1/2, chunk 64..1024) and return the fastest.  Results are cached per
+ * device name and batch signature in-process and, when cache_path is not NULL,
+ * in that text file.
+ */
+int vrt_batch_autotune(const vrt_world* world, const vrt_batch* batch, const char* cache_path,
+                       vrt_batch_tuning* best, float* best_ms, void* stream);
+
+/* ---------------------------------------------------------------- differentiable */
+
+/*
+ * Lambertian render of camera views, one primary ray per pixel, no shadows:
+ *   radiance_c = albedo[v][c] * max(0, dot(N, L)) * intensity   (0 on a miss)
+ * albedo is fp32 RGB per voxel in slot order: index (slot * 512 + voxel) * 3 + c.
+ * image: [sum of width*height over views] * 3, views concatenated, pixels in
+ * each camera's ray order.  saved_voxel (u32, ~0 on a miss) and saved_weight
+ * (f32) are per pixel and are consumed by the backward pass.
+ */
+int vrt_render_fwd(const vrt_world* world, const vrt_camera* cameras, uint32_t nviews,
+                   uint32_t max_iters, const float light_dir[3], float intensity,
+                   const float* albedo, float* image, uint32_t* saved_voxel,
+                   float* saved_weight, void* stream);
+
+/* grad_albedo[v][c] += sum over pixels hitting v of grad_image[p][c] * saved_weight[p]. */
+int vrt_render_bwd(const vrt_world* world, uint32_t npixels, const float* grad_image,
+                   const uint32_t* saved_voxel, const float* saved_weight,
+                   float* grad_albedo, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VRT_H */

@@ -1,0 +1,102 @@
+/*
+ * vrt_scene.h - host-side synthetic worlds and rays for libvoxelrt_hip.so.
+ *
+ * These replace the reference's world producers (TerrainGenerator,
+ * src/VoxelRT/TerrainGenerator.cpp:5-146, FastNoise2-based) with seeded,
+ * fully specified generators, and its ray producers (GetPrimaryRay,
+ * src/VoxelRT/CpuRenderer.cpp:226-233) with a pinhole camera built from
+ * IEEE operations only, so host and device generate identical rays.
+ * All pointers are host pointers; nothing here touches the GPU.
+ */
+#ifndef VRT_SCENE_H
+#define VRT_SCENE_H
+
+#include <stdint.h>
+
+#include "vrt.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/*
+ * A world as a sector stream (the input format of vrt_world_sync):
+ * sectors in increasing index, each with its allocation mask and
+ * popcount(mask) bricks of 512 bytes in increasing brick index.
+ * Host memory owned by the vrt_scene.
+ */
+typedef struct vrt_sector_stream {
+    uint32_t log2_size_xz, log2_size_y;
+    uint32_t nsec;
+    const uint32_t* sector_idx;
+    const uint64_t* alloc_mask;
+    const uint8_t* bricks;
+    uint64_t nbricks;
+    uint64_t solid_voxels;
+} vrt_sector_stream;
+
+typedef struct vrt_scene vrt_scene;
+
+/*
+ * Heightmap terrain.  Height H(x, z) = base + amp * fbm2(x, z), fbm2 = value
+ * noise fBm (octaves, frequency, gain, lacunarity 2) normalised to [0, 1).
+ * A voxel (x, y, z) is solid when y < H(x, z) and, if cave_threshold > 0, the
+ * 3D value-noise fBm (cave_octaves, cave_frequency, gain 0.5) at the voxel
+ * centre is above cave_threshold.  Material = 1 + (y / band) % 255.
+ */
+typedef struct vrt_terrain_params {
+    uint32_t log2_size_xz, log2_size_y;
+    uint64_t seed;
+    uint32_t octaves;
+    float frequency;
+    float gain;
+    float base, amp;          /* in voxels */
+    float cave_threshold;     /* 0 = no caves */
+    uint32_t cave_octaves;
+    float cave_frequency;
+    uint32_t band;            /* material band height in voxels */
+} vrt_terrain_params;
+
+int vrt_scene_terrain(const vrt_terrain_params* p, uint32_t threads, vrt_scene** out);
+
+/*
+ * Spheres: `count` centres uniform in [0, size), radius uniform in
+ * [rmin, rmax), material 1 + (hash % 255); a voxel belongs to a sphere when
+ * its centre is within the radius; later spheres overwrite earlier ones.
+ */
+int vrt_scene_spheres(uint32_t log2_size_xz, uint32_t log2_size_y, uint32_t count, float rmin, float rmax,
+                      uint64_t seed, uint32_t threads, vrt_scene** out);
+
+/* A scene from a dense voxel grid [y][z][x] of size_y * size_xz * size_xz bytes. */
+int vrt_scene_from_dense(uint32_t log2_size_xz, uint32_t log2_size_y, const uint8_t* voxels, vrt_scene** out);
+
+int vrt_scene_get(const vrt_scene* scene, vrt_sector_stream* out);
+/* Voxel at absolute coordinates (0 outside the world). */
+uint8_t vrt_scene_voxel(const vrt_scene* scene, int32_t x, int32_t y, int32_t z);
+void vrt_scene_destroy(vrt_scene* scene);
+
+/* Convenience: vrt_world_create + vrt_world_sync of the whole scene. */
+int vrt_world_from_scene(const vrt_scene* scene, void* stream, vrt_world** out);
+
+/*
+ * Camera set-up: eye at `pos` (absolute, double), yaw/pitch in degrees
+ * (yaw 0 looks along +z, pitch > 0 looks up), vertical field of view fov_deg.
+ * world_origin = floor(pos), eye = pos - world_origin.
+ */
+int vrt_camera_look(const double pos[3], double yaw_deg, double pitch_deg, double fov_deg, uint32_t width,
+                    uint32_t height, uint32_t tile_w, uint32_t tile_h, vrt_camera* out);
+
+/* Host generation of a camera's rays (bit-identical to vrt_camera_rays). out: 6 arrays of width*height floats. */
+int vrt_camera_rays_host(const vrt_camera* cam, float* const out[6]);
+
+/* n rays with origins uniform in [0, size) and directions uniform on the sphere; world_origin 0. */
+int vrt_incoherent_rays_host(uint32_t n, uint32_t size_xz, uint32_t size_y, uint64_t seed, float* const out[6]);
+
+/* Host twin of vrt_bounce_generate for checking it.  Returns the bounce count via *count. */
+int vrt_bounce_host(uint32_t n, const vrt_hit* rec, const float* pos /* 3*n SoA */, uint32_t width,
+                    uint32_t seed, float* const out[6], uint32_t* src, uint32_t* count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,124 @@
+/*
+ * oracle.h - CPU restatement of VoxelRT's brickmap ray traversal.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP path and
+ * the timed CPU baseline; it is never linked into, loaded by or called from
+ * the product library.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * CPU-baseline leg use it.
+ *
+ * World layout: the reference's FlatVoxelStorage (src/VoxelRT/CpuRenderer.cpp:20-84)
+ * with the view extents as parameters instead of the fixed 2048x512x2048:
+ *   SectorMasks[nsec]                 u64, bit = brick allocated
+ *   StorageBuffer[nsec * 32768]       u8, sector * 32 KiB + brick * 512 + voxel
+ *   OccupancyStorage[nsec * 512]      u64, sector * 512 + brick * 8 + cell
+ * i.e. dense per sector, unlike the product's popcount-compacted slots, so the
+ * two layouts check each other.
+ *
+ * Traversal contract (both the scalar and the SIMD restatement):
+ *   FP32, IEEE division for 1/d, explicit fmaf where the reference's
+ *   -ffast-math build contracts, x86 min semantics (a < b ? a : b), x86
+ *   cvtps2dq floor2i semantics (NaN / out of range -> INT_MIN), wrapping int
+ *   arithmetic, un-biased hit distance, absolute hit voxel for the material.
+ *   Build with -ffp-contract=off and without -ffast-math.
+ */
+#ifndef VRT_ORACLE_H
+#define VRT_ORACLE_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct oracle_world {
+    uint32_t shift_xz, shift_y;   /* log2 of the sector grid extents */
+    uint32_t size_xz, size_y;     /* voxel extents */
+    uint32_t nsec;
+    uint64_t* sector_masks;       /* [nsec] */
+    uint8_t* storage;             /* [nsec * 32768] */
+    uint64_t* occupancy;          /* [nsec * 512] */
+    uint64_t palette[256];
+} oracle_world;
+
+/* Hit record in the same packing as the product's vrt_hit (include/vrt.h). */
+typedef struct oracle_hit {
+    int32_t x, y, z;
+    uint32_t info;      /* material | normal code << 8 | iterations << 16 */
+} oracle_hit;
+
+typedef struct oracle_counters {
+    uint64_t rays, iters, descents, hits;
+} oracle_counters;
+
+/* Build from a sector stream: for each sector, its allocation mask and
+ * popcount(mask) bricks of 512 bytes in increasing brick index. */
+int oracle_world_create(uint32_t log2_size_xz, uint32_t log2_size_y, uint32_t nsec_in,
+                        const uint32_t* sector_idx, const uint64_t* alloc_mask,
+                        const uint8_t* bricks, oracle_world** out);
+void oracle_world_destroy(oracle_world* w);
+void oracle_world_set_palette(oracle_world* w, const uint64_t* palette);
+uint8_t oracle_world_get_voxel(const oracle_world* w, int32_t x, int32_t y, int32_t z);
+
+/* Scalar cast of rays [begin, end). t[i] = -1 on a miss. pos may be NULL. descents may be NULL. */
+void oracle_cast(const oracle_world* w, const float* const o[3], const float* const d[3],
+                 const int32_t world_origin[3], uint32_t begin, uint32_t end, uint32_t max_iters,
+                 float* t, oracle_hit* rec, float* const pos[3], uint16_t* descents);
+
+/* Counter totals of a scalar cast (same definitions as the product's counters). */
+void oracle_count(const oracle_world* w, const float* const o[3], const float* const d[3],
+                  const int32_t world_origin[3], uint32_t n, uint32_t max_iters,
+                  oracle_counters* out);
+
+/* SIMD restatement of the reference's vector loop (CpuRenderer.cpp:172-224):
+ * 16 lanes on AVX-512, else 8 lanes on AVX2, consecutive rays per vector,
+ * `threads` worker threads (0 = all cores).  Results identical to oracle_cast.
+ * Returns the lane width used. */
+int oracle_cast_simd(const oracle_world* w, const float* const o[3], const float* const d[3],
+                     const int32_t world_origin[3], uint32_t n, uint32_t max_iters,
+                     uint32_t threads, float* t, oracle_hit* rec);
+/* Lane width oracle_cast_simd would use on this host (16, 8, or 0 if unsupported). */
+int oracle_simd_width(void);
+int oracle_num_cores(void);
+
+/* Scalar FP64 cast of VoxelMap::RayCast (VoxelMap.cpp:125-170). distance <= 0: miss. */
+typedef struct oracle_hit_f64 {
+    double distance;
+    float normal[3];
+    float uv[2];
+    int32_t voxel[3];
+} oracle_hit_f64;
+void oracle_cast_f64(const oracle_world* w, const double origin[3], const double dir[3],
+                     uint32_t max_iters, oracle_hit_f64* out);
+
+/* Brute-force FP64 Amanatides-Woo DDA without LOD: first solid voxel along the
+ * ray from (world_origin + o), up to max_steps voxel steps.  Returns 1 on a hit
+ * and writes the voxel and the distance of the entry face; also returns the
+ * smallest gap between the two nearest face crossings seen (ambiguity margin). */
+int oracle_dda_f64(const oracle_world* w, const double o[3], const double d[3],
+                   const int32_t world_origin[3], uint32_t max_steps,
+                   int32_t voxel[3], double* t, double* min_gap);
+
+/* PCG of RandomGen.glsl:6-20. */
+uint32_t oracle_pcg_init(uint32_t px, uint32_t py, uint32_t seed);
+uint32_t oracle_pcg_next(uint32_t* state);
+/* sin and cos of 2*pi*u for u in [0, 1]. */
+void oracle_sincos_turns(float u, float* s, float* c);
+
+/* Bounce construction (CpuRenderer.cpp:389-392 with the PCG sample of
+ * VoxelRender.comp:71): compacted bounce rays of the hits, in increasing i. */
+uint32_t oracle_bounce(uint32_t n, const float* t, const oracle_hit* rec, const float* const pos[3],
+                       const float* const d[3], uint32_t width, uint32_t seed,
+                       float* const out_o[3], float* const out_d[3], uint32_t* src);
+
+/* Reference gradient of L = sum_p sum_c (radiance - target)^2 for the Lambertian
+ * render, FP64: grad[v*3+c] += 2 (radiance_pc - target_pc) * weight_p. */
+void oracle_render_grad(uint32_t npix, const uint32_t* saved_voxel, const float* saved_weight,
+                        const float* albedo, const float* target, double* grad);
+double oracle_render_loss(uint32_t npix, const uint32_t* saved_voxel, const float* saved_weight,
+                          const double* albedo, const float* target);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

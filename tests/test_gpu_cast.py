@@ -1,0 +1,141 @@
+"""GPU parity of the plain per-ray cast (vrt_cast through the C ABI) against the oracle."""
+from __future__ import annotations
+
+from pathlib import Path
+
+import numpy as np
+import pytest
+import torch
+
+from helpers import assert_hits_equal, blob_world, random_rays
+from test_oracle import dense_world
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+SCHEDULES = [(1, 0), (2, 64), (2, 256), (2, 1024)]  # (schedule, chunk)
+
+
+def gpu_cast(world, o, d, W, max_iters=128, schedule=1, chunk=0, with_pos=False, counters=None):
+    from voxelrt_amd import RayBatch
+
+    rb = RayBatch.from_numpy(o, d, W)
+    h = world.cast(rb, max_iters, with_pos=with_pos, schedule=schedule, chunk=chunk, counters=counters)
+    torch.cuda.synchronize()
+    return h.t.cpu().numpy(), h.rec.cpu().numpy(), (h.pos.cpu().numpy() if h.pos is not None else None)
+
+
+def pair_from_dense(vrt, oracle_mod, vox):
+    sc = vrt.Scene.from_dense(vox)
+    return vrt.World.from_scene(sc), dense_world(oracle_mod, vox)
+
+
+def test_golden_fixture_on_gpu(vrt, oracle_mod, cuda):
+    f = np.load(GOLDEN / "blob64.npz")
+    world, _ = pair_from_dense(vrt, oracle_mod, f["voxels"])
+    for sched, chunk in SCHEDULES:
+        t, rec, _ = gpu_cast(world, f["o"], f["d"], tuple(int(x) for x in f["world_origin"]), int(f["max_iters"]),
+                             sched, chunk)
+        assert_hits_equal(t, rec, f["t"], f["rec"], f"golden sched={sched} chunk={chunk}")
+        assert np.array_equal(t.view(np.uint32), f["t"].view(np.uint32))
+
+
+def test_c1_terrain_parity(vrt, oracle_mod, cuda):
+    from voxelrt_amd import workloads as W
+    import oracle
+
+    sc = W.C1_WORLD.build()
+    world = vrt.World.from_scene(sc)
+    ow = oracle.OracleWorld.from_stream(sc.stream())
+    cam = W.c1_camera()
+    o, d = cam.rays_host()
+    ref = ow.cast(o, d, cam.world_origin, W.MAX_ITERS, with_pos=True)
+    for sched, chunk in SCHEDULES:
+        t, rec, pos = gpu_cast(world, o, d, cam.world_origin, W.MAX_ITERS, sched, chunk, with_pos=True)
+        assert_hits_equal(t, rec, ref["t"], ref["rec"], f"C1 sched={sched}")
+        assert np.array_equal(t.view(np.uint32), ref["t"].view(np.uint32)), "t not bit-exact"
+        hit = (rec[:, 3] & 0xFF) != 0
+        assert np.array_equal(pos[:, hit].view(np.uint32), ref["pos"][:, hit].view(np.uint32))
+    assert ((ref["rec"][:, 3] & 0xFF) != 0).mean() > 0.2
+
+
+@pytest.mark.parametrize("seed,size,inside,max_iters", [(1, 64, True, 128), (2, 64, False, 128),
+                                                        (3, 128, True, 512), (4, 128, False, 40)])
+def test_random_world_parity(vrt, oracle_mod, cuda, seed, size, inside, max_iters):
+    vox = blob_world(size, seed=seed, density=0.01)
+    world, ow = pair_from_dense(vrt, oracle_mod, vox)
+    o, d = random_rays(30000, size, seed=seed + 50, inside=inside)
+    W = (0, 0, 0) if seed % 2 else (-7, 3, 11)
+    o = (o - np.array(W, np.float32)[:, None]).astype(np.float32)
+    ref = ow.cast(o, d, W, max_iters)
+    for sched, chunk in SCHEDULES:
+        t, rec, _ = gpu_cast(world, o, d, W, max_iters, sched, chunk)
+        assert_hits_equal(t, rec, ref["t"], ref["rec"], f"seed={seed} sched={sched}")
+        assert np.array_equal(t.view(np.uint32), ref["t"].view(np.uint32))
+
+
+def test_special_rays(vrt, oracle_mod, cuda):
+    """Exact zeros (inf/NaN side distances), denormal and huge components, far origins (floor2i
+    out of range), origins on voxel faces, empty and solid worlds."""
+    vox = blob_world(64, seed=9, density=0.02)
+    full = np.full((64, 64, 64), 5, np.uint8)
+    empty = np.zeros((64, 64, 64), np.uint8)
+    rng = np.random.default_rng(1)
+    n = 4096
+    o = rng.uniform(-4, 68, (3, n)).astype(np.float32)
+    o[:, :512] = np.floor(o[:, :512])                       # on voxel corners
+    o[0, 512:600] = np.float32(3e9)                         # floor2i out of range
+    o[1, 600:700] = np.float32(-3e9)
+    o[2, 700:720] = np.float32(np.nan)
+    d = rng.normal(size=(3, n)).astype(np.float32)
+    d[1, 1000:1500] = 0.0                                   # exact zeros
+    d[0, 1500:1800] = 0.0
+    d[2, 1500:1800] = 0.0
+    d[0, 1800:2000] = np.float32(1e-40)                     # denormal
+    d[2, 2000:2100] = np.float32(-1e-41)
+    d[1, 2100:2200] = np.float32(1e30)                      # huge
+    d[:, 2200:2300] = np.float32(-0.0)
+    d[0, 2200:2300] = 1.0
+    for v in (vox, full, empty):
+        world, ow = pair_from_dense(vrt, oracle_mod, v)
+        ref = ow.cast(o, d, (0, 0, 0), 128)
+        for sched, chunk in SCHEDULES[:2]:
+            t, rec, _ = gpu_cast(world, o, d, (0, 0, 0), 128, sched, chunk)
+            assert_hits_equal(t, rec, ref["t"], ref["rec"], f"special sched={sched}")
+            assert np.array_equal(t.view(np.uint32), ref["t"].view(np.uint32))
+
+
+def test_counters_match_oracle(vrt, oracle_mod, cuda):
+    vox = blob_world(128, seed=12, density=0.01)
+    world, ow = pair_from_dense(vrt, oracle_mod, vox)
+    o, d = random_rays(20000, 128, seed=13)
+    ref = ow.count(o, d, (0, 0, 0), 128)
+    for sched, chunk in SCHEDULES:
+        ctr = torch.zeros(4, dtype=torch.int64, device="cuda")
+        gpu_cast(world, o, d, (0, 0, 0), 128, sched, chunk, counters=ctr)
+        c = ctr.cpu().numpy()
+        assert (c[0], c[1], c[2], c[3]) == (ref["rays"], ref["iters"], ref["descents"], ref["hits"])
+
+
+def test_device_camera_rays_equal_host(vrt, cuda):
+    from voxelrt_amd import workloads as W
+
+    for tile in [(1, 1), (8, 8), (16, 4)]:
+        cam = W.c2_camera(tile=tile, width=640, height=360)
+        ho, hd = cam.rays_host()
+        rb = cam.rays_device()
+        torch.cuda.synchronize()
+        assert np.array_equal(rb.o.cpu().numpy().view(np.uint32), ho.view(np.uint32))
+        assert np.array_equal(rb.d.cpu().numpy().view(np.uint32), hd.view(np.uint32))
+
+
+def test_fp64_picking_cast_matches_oracle(vrt, oracle_mod, cuda):
+    vox = blob_world(64, seed=7, density=0.004, blobs=3)
+    world, ow = pair_from_dense(vrt, oracle_mod, vox)
+    o, d = random_rays(300, 64, seed=78, inside=False)
+    for i in range(o.shape[1]):
+        oo, dd = o[:, i].astype(np.float64), d[:, i].astype(np.float64)
+        g = world.ray_cast(oo, dd, 1024)
+        r = ow.cast_f64(oo, dd, 1024)
+        assert g.distance == r["distance"] and g.voxel_pos == r["voxel"] and g.normal == r["normal"]
+        assert g.uv == r["uv"]

@@ -1,0 +1,325 @@
+"""Python mirror of the reference's voxel-map and ray-cast interface.
+
+    reference (src/VoxelRT)                        here
+    VoxelMap + TerrainGenerator (host world)       Scene (seeded generators, sector stream)
+    GpuVoxelStorage::SyncBuffers (device world)    World.sync / World.from_scene
+    RayCast(storage, origin, dir, mask, W)         World.cast(RayBatch) -> HitBatch
+    HitResult VoxelMap::RayCast(dvec3, dvec3, n)   World.ray_cast(origin, dir, max_iters=1024)
+
+Device memory is held in torch tensors (plumbing only); all traversal work is
+done by libvoxelrt_hip.so through its C ABI.  Every failure raises VrtError.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib as L
+
+
+def _stream_handle(stream) -> int | None:
+    if stream is None:
+        return torch.cuda.current_stream().cuda_stream
+    if isinstance(stream, torch.cuda.Stream):
+        return stream.cuda_stream
+    return int(stream)
+
+
+def _ptr(t) -> int:
+    if t is None:
+        return 0
+    if isinstance(t, np.ndarray):
+        return t.ctypes.data
+    return t.data_ptr()
+
+
+# ------------------------------------------------------------------ scenes
+
+
+class Scene:
+    """Host world as a sector stream (vrt_scene)."""
+
+    def __init__(self, handle: int):
+        self._h = C.c_void_p(handle)
+
+    @staticmethod
+    def terrain(log2_xz: int, log2_y: int, seed: int, *, octaves=4, frequency=1 / 64, gain=0.5, base=32.0,
+                amp=128.0, cave_threshold=0.0, cave_octaves=3, cave_frequency=1 / 32, band=16,
+                threads=0) -> "Scene":
+        p = L.TerrainParams(log2_xz, log2_y, seed, octaves, frequency, gain, base, amp, cave_threshold,
+                            cave_octaves, cave_frequency, band)
+        out = C.c_void_p()
+        L.check(L.lib().vrt_scene_terrain(C.byref(p), threads, C.byref(out)))
+        return Scene(out.value)
+
+    @staticmethod
+    def spheres(log2_xz: int, log2_y: int, count: int, rmin: float, rmax: float, seed: int, threads=0) -> "Scene":
+        out = C.c_void_p()
+        L.check(L.lib().vrt_scene_spheres(log2_xz, log2_y, count, rmin, rmax, seed, threads, C.byref(out)))
+        return Scene(out.value)
+
+    @staticmethod
+    def from_dense(voxels: np.ndarray) -> "Scene":
+        """voxels: uint8 [Y][Z][X], powers of two >= 32."""
+        v = np.ascontiguousarray(voxels, dtype=np.uint8)
+        sy, sz, sx = v.shape
+        assert sx == sz, "x and z extents must match"
+        out = C.c_void_p()
+        L.check(L.lib().vrt_scene_from_dense(int(sx).bit_length() - 1, int(sy).bit_length() - 1,
+                                             v.ctypes.data, C.byref(out)))
+        return Scene(out.value)
+
+    def stream(self) -> dict:
+        s = L.SectorStream()
+        L.check(L.lib().vrt_scene_get(self._h, C.byref(s)))
+
+        def view(ptr, n, dt):
+            if n == 0:
+                return np.zeros(0, dtype=dt)
+            buf = (C.c_uint8 * (n * np.dtype(dt).itemsize)).from_address(ptr)
+            return np.frombuffer(buf, dtype=dt)
+
+        return {
+            "log2_size_xz": s.log2_size_xz,
+            "log2_size_y": s.log2_size_y,
+            "nsec": s.nsec,
+            "sector_idx": view(s.sector_idx, s.nsec, np.uint32),
+            "alloc_mask": view(s.alloc_mask, s.nsec, np.uint64),
+            "bricks": view(s.bricks, int(s.nbricks) * 512, np.uint8),
+            "nbricks": int(s.nbricks),
+            "solid_voxels": int(s.solid_voxels),
+        }
+
+    def voxel(self, x: int, y: int, z: int) -> int:
+        return int(L.lib().vrt_scene_voxel(self._h, x, y, z))
+
+    def close(self) -> None:
+        if self._h and self._h.value:
+            L.lib().vrt_scene_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------------ rays and hits
+
+
+@dataclass
+class RayBatch:
+    """n rays, SoA float32 device tensors; origins relative to world_origin."""
+
+    o: torch.Tensor          # [3, n]
+    d: torch.Tensor          # [3, n]
+    world_origin: tuple
+
+    @property
+    def n(self) -> int:
+        return int(self.o.shape[1])
+
+    def to_c(self) -> L.Rays:
+        o, d = self.o, self.d
+        r = L.Rays(o[0].data_ptr(), o[1].data_ptr(), o[2].data_ptr(), d[0].data_ptr(), d[1].data_ptr(),
+                   d[2].data_ptr(), (C.c_int32 * 3)(*self.world_origin), self.n)
+        return r
+
+    @staticmethod
+    def from_numpy(o: np.ndarray, d: np.ndarray, world_origin=(0, 0, 0), device="cuda") -> "RayBatch":
+        return RayBatch(torch.from_numpy(np.ascontiguousarray(o, dtype=np.float32)).to(device),
+                        torch.from_numpy(np.ascontiguousarray(d, dtype=np.float32)).to(device),
+                        tuple(int(x) for x in world_origin))
+
+    @staticmethod
+    def empty(n: int, world_origin=(0, 0, 0), device="cuda") -> "RayBatch":
+        return RayBatch(torch.empty((3, n), dtype=torch.float32, device=device),
+                        torch.empty((3, n), dtype=torch.float32, device=device),
+                        tuple(int(x) for x in world_origin))
+
+    def buffers(self) -> L.RayBuffers:
+        o, d = self.o, self.d
+        return L.RayBuffers(o[0].data_ptr(), o[1].data_ptr(), o[2].data_ptr(), d[0].data_ptr(), d[1].data_ptr(),
+                            d[2].data_ptr())
+
+
+@dataclass
+class HitBatch:
+    t: torch.Tensor          # [n] float32, -1 on a miss
+    rec: torch.Tensor        # [n, 4] int32: x, y, z, info
+    pos: torch.Tensor | None  # [3, n] float32
+
+    @staticmethod
+    def empty(n: int, with_pos=False, device="cuda") -> "HitBatch":
+        return HitBatch(torch.empty(n, dtype=torch.float32, device=device),
+                        torch.empty((n, 4), dtype=torch.int32, device=device),
+                        torch.empty((3, n), dtype=torch.float32, device=device) if with_pos else None)
+
+    def to_c(self) -> L.Hits:
+        return L.Hits(self.t.data_ptr(), self.rec.data_ptr(), self.pos.data_ptr() if self.pos is not None else 0)
+
+    # decoding of vrt_hit.info
+    def material(self) -> torch.Tensor:
+        return self.rec[:, 3] & 0xFF
+
+    def iterations(self) -> torch.Tensor:
+        return (self.rec[:, 3] >> 16) & 0xFFFF
+
+    def normal(self) -> torch.Tensor:
+        info = self.rec[:, 3]
+        comps = [((info >> (8 + 2 * a)) & 3) for a in range(3)]
+        return torch.stack([(c ^ 2) - 2 for c in comps], dim=1)
+
+
+@dataclass
+class HitResult:
+    """VoxelMap::HitResult (VoxelMap.h:171-178)."""
+
+    distance: float
+    normal: tuple
+    uv: tuple
+    voxel_pos: tuple
+
+    def is_miss(self) -> bool:
+        return self.distance <= 0.0
+
+
+class CameraSpec:
+    """vrt_camera built by vrt_camera_look."""
+
+    def __init__(self, pos, yaw_deg: float, pitch_deg: float, width: int, height: int, fov_deg: float = 90.0,
+                 tile=(1, 1)):
+        cam = L.Camera()
+        L.check(L.lib().vrt_camera_look(C.byref((C.c_double * 3)(*pos)), yaw_deg, pitch_deg, fov_deg, width,
+                                        height, tile[0], tile[1], C.byref(cam)))
+        self.c = cam
+
+    @property
+    def n(self) -> int:
+        return self.c.width * self.c.height
+
+    @property
+    def world_origin(self) -> tuple:
+        return tuple(self.c.world_origin)
+
+    def rays_device(self, device="cuda", stream=None) -> RayBatch:
+        rb = RayBatch.empty(self.n, self.world_origin, device)
+        bufs = rb.buffers()
+        L.check(L.lib().vrt_camera_rays(C.byref(self.c), C.byref(bufs), _stream_handle(stream)))
+        return rb
+
+    def rays_host(self) -> tuple[np.ndarray, np.ndarray]:
+        out = np.empty((6, self.n), dtype=np.float32)
+        ptrs = (C.c_void_p * 6)(*[out[i].ctypes.data for i in range(6)])
+        L.check(L.lib().vrt_camera_rays_host(C.byref(self.c), C.byref(ptrs)))
+        return out[:3].copy(), out[3:].copy()
+
+
+def incoherent_rays_host(n: int, size_xz: int, size_y: int, seed: int) -> tuple[np.ndarray, np.ndarray]:
+    out = np.empty((6, n), dtype=np.float32)
+    ptrs = (C.c_void_p * 6)(*[out[i].ctypes.data for i in range(6)])
+    L.check(L.lib().vrt_incoherent_rays_host(n, size_xz, size_y, seed, C.byref(ptrs)))
+    return out[:3].copy(), out[3:].copy()
+
+
+# ------------------------------------------------------------------ device world
+
+
+class World:
+    """Device brickmap (vrt_world)."""
+
+    def __init__(self, log2_xz: int, log2_y: int, initial_slots: int = 0, max_slots: int = 0, stream=None):
+        desc = L.WorldDesc(log2_xz, log2_y, initial_slots, max_slots)
+        out = C.c_void_p()
+        L.check(L.lib().vrt_world_create(C.byref(desc), _stream_handle(stream), C.byref(out)))
+        self._h = out
+
+    @staticmethod
+    def from_scene(scene: Scene, stream=None) -> "World":
+        out = C.c_void_p()
+        L.check(L.lib().vrt_world_from_scene(scene._h, _stream_handle(stream), C.byref(out)))
+        w = World.__new__(World)
+        w._h = out
+        return w
+
+    @property
+    def handle(self) -> C.c_void_p:
+        return self._h
+
+    def sync(self, sector_idx: np.ndarray, alloc_mask: np.ndarray, bricks: np.ndarray, stream=None) -> None:
+        si = np.ascontiguousarray(sector_idx, dtype=np.uint32)
+        am = np.ascontiguousarray(alloc_mask, dtype=np.uint64)
+        br = np.ascontiguousarray(bricks, dtype=np.uint8)
+        L.check(L.lib().vrt_world_sync(self._h, len(si), si.ctypes.data, am.ctypes.data,
+                                       br.ctypes.data if br.size else 0, _stream_handle(stream)))
+
+    def set_palette(self, palette: np.ndarray, stream=None) -> None:
+        p = np.ascontiguousarray(palette, dtype=np.uint64)
+        assert p.shape == (256,)
+        L.check(L.lib().vrt_world_set_palette(self._h, p.ctypes.data, _stream_handle(stream)))
+
+    def info(self) -> L.WorldInfo:
+        i = L.WorldInfo()
+        L.check(L.lib().vrt_world_get_info(self._h, C.byref(i)))
+        return i
+
+    def buffers(self) -> L.WorldBuffers:
+        b = L.WorldBuffers()
+        L.check(L.lib().vrt_world_get_buffers(self._h, C.byref(b)))
+        return b
+
+    def cast(self, rays: RayBatch, max_iters: int = 128, with_pos: bool = False, schedule: int = 0,
+             chunk: int = 0, waves_per_cu: int = 0, counters: torch.Tensor | None = None,
+             out: HitBatch | None = None, stream=None) -> HitBatch:
+        hits = out if out is not None else HitBatch.empty(rays.n, with_pos, rays.o.device)
+        r = rays.to_c()
+        h = hits.to_c()
+        opts = L.CastOpts(schedule, 1, chunk, waves_per_cu, counters.data_ptr() if counters is not None else 0)
+        L.check(L.lib().vrt_cast_ex(self._h, C.byref(r), max_iters, C.byref(h), C.byref(opts),
+                                    _stream_handle(stream)))
+        return hits
+
+    def ray_cast(self, origin, direction, max_iters: int = 1024) -> HitResult:
+        """HitResult VoxelMap::RayCast(dvec3 origin, dvec3 dir, uint32_t maxIters = 1024) (VoxelMap.h:214)."""
+        o = (C.c_double * 3)(*origin)
+        d = (C.c_double * 3)(*direction)
+        h = L.HitF64()
+        L.check(L.lib().vrt_cast_one_f64(self._h, C.byref(o), C.byref(d), max_iters, C.byref(h)))
+        return HitResult(h.distance, tuple(h.normal), tuple(h.uv), tuple(h.voxel))
+
+    def close(self) -> None:
+        if self._h and self._h.value:
+            L.check(L.lib().vrt_world_destroy(self._h))
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def new_counters(device="cuda") -> torch.Tensor:
+    """Zeroed vrt_cast_counters on the device (rays, iters, descents, hits)."""
+    return torch.zeros(4, dtype=torch.int64, device=device)
+
+
+def bounce(rays: RayBatch, hits: HitBatch, width: int, seed: int, stream=None):
+    """Diffuse bounce rays of the hits (vrt_bounce_generate); returns (RayBatch, src, count tensor)."""
+    assert hits.pos is not None, "bounce generation needs hit positions"
+    n = rays.n
+    dev = rays.o.device
+    out = RayBatch.empty(n, rays.world_origin, dev)
+    src = torch.empty(n, dtype=torch.int32, device=dev)
+    count = torch.zeros(1, dtype=torch.int32, device=dev)
+    scratch = torch.empty(int(L.lib().vrt_bounce_scratch_bytes(n)) // 4 + 1, dtype=torch.int32, device=dev)
+    r = rays.to_c()
+    h = hits.to_c()
+    bufs = out.buffers()
+    L.check(L.lib().vrt_bounce_generate(C.byref(r), C.byref(h), width, seed, C.byref(bufs), src.data_ptr(),
+                                        count.data_ptr(), scratch.data_ptr(), _stream_handle(stream)))
+    return out, src, count

@@ -1,0 +1,385 @@
+// cast.hip - plain per-ray cast kernels and their C ABI (vrt_cast, vrt_cast_ex,
+// vrt_cast_one_f64).
+//
+// vrt_cast replaces the reference's RayCast (src/VoxelRT/CpuRenderer.cpp:172-224)
+// for a buffer of rays; vrt_cast_one_f64 replaces VoxelMap::RayCast
+// (src/VoxelRT/VoxelMap.cpp:140-170).  Two schedules compute identical results:
+//   GRID        one lane per ray, the grid covers n rays;
+//   PERSISTENT  a grid sized to the device; each wave takes chunks of rays from
+//               a global counter into a wave-local pool and refills lanes whose
+//               ray finished (ballot + mbcnt rank), so long rays do not keep
+//               the other 63 lanes of their wave idle.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cstring>
+#include <mutex>
+
+#include "traverse.h"
+#include "vrt_cast_impl.h"
+
+namespace vrt {
+
+// ---------------------------------------------------------------- counters
+
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long x) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    return x;
+}
+
+__device__ __forceinline__ void flush_counters(vrt_cast_counters* ctr, unsigned long long rays, unsigned long long iters,
+                                               unsigned long long desc, unsigned long long hits) {
+    rays = wave_sum(rays);
+    iters = wave_sum(iters);
+    desc = wave_sum(desc);
+    hits = wave_sum(hits);
+    if (__lane_id() == 0) {
+        atomicAdd(&ctr->rays, rays);
+        atomicAdd(&ctr->iters, iters);
+        atomicAdd(&ctr->descents, desc);
+        atomicAdd(&ctr->hits, hits);
+    }
+}
+
+__device__ __forceinline__ void store_result(const HitsOut& h, uint32_t i, const RayState& r, float t, int4 rec) {
+    h.t[i] = t;
+    h.rec[i] = rec;
+    if (h.pos_x) {
+        h.pos_x[i] = rec.w & 0xFF ? r.cur[0] : 0.0f;
+        h.pos_y[i] = rec.w & 0xFF ? r.cur[1] : 0.0f;
+        h.pos_z[i] = rec.w & 0xFF ? r.cur[2] : 0.0f;
+    }
+}
+
+// Traces one ray to completion.
+__device__ __forceinline__ void trace_full(const WorldView& w, RayState& st, uint32_t max_iters, float& t, int4& rec) {
+    int32_t v[3];
+    uint32_t slot = 0;
+    int res = STEP_MISS;
+    for (uint32_t it = 0; it < max_iters; it++) {
+        res = ray_step(w, st, v, &slot);
+        if (res != STEP_CONTINUE) break;
+    }
+    if (res == STEP_HIT)
+        ray_hit_record(w, st, v, slot, t, rec);
+    else
+        ray_miss_record(st, t, rec);
+}
+
+// ---------------------------------------------------------------- GRID schedule
+
+template <bool COUNT>
+__global__ __launch_bounds__(256) void cast_grid_kernel(WorldView w, RaysIn r, HitsOut h, uint32_t max_iters,
+                                                        vrt_cast_counters* ctr) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool valid = i < r.n;
+    RayState st;
+    float t = 0.0f;
+    int4 rec = make_int4(0, 0, 0, 0);
+    if (valid) {
+        ray_setup(st, r.ox[i], r.oy[i], r.oz[i], r.dx[i], r.dy[i], r.dz[i], r.W[0], r.W[1], r.W[2]);
+        trace_full(w, st, max_iters, t, rec);
+        store_result(h, i, st, t, rec);
+    }
+    if constexpr (COUNT) {
+        bool hit = valid && (rec.w & 0xFF) != 0;
+        flush_counters(ctr, valid ? 1ull : 0ull, valid ? st.iters : 0ull, valid ? st.descents : 0ull, hit ? 1ull : 0ull);
+    }
+}
+
+// ---------------------------------------------------------------- PERSISTENT schedule
+
+template <bool COUNT>
+__global__ __launch_bounds__(256) void cast_persistent_kernel(WorldView w, RaysIn r, HitsOut h, uint32_t max_iters,
+                                                              uint32_t chunk, uint32_t refill_min, uint32_t* queue,
+                                                              vrt_cast_counters* ctr) {
+    const uint32_t n = r.n;
+    uint32_t pool_next = 0, pool_end = 0;  // wave-uniform
+    bool exhausted = false;                // wave-uniform: the global queue is drained
+    bool has = false;
+    uint32_t idx = 0;
+    RayState st;
+    unsigned long long c_rays = 0, c_iters = 0, c_desc = 0, c_hits = 0;
+    for (;;) {
+        uint64_t idle = __ballot(!has);
+        uint32_t nidle = (uint32_t)__popcll(idle);
+        if (nidle >= refill_min || nidle == 64) {
+            if (pool_next == pool_end && !exhausted) {
+                uint32_t g = 0;
+                if (__lane_id() == 0) g = atomicAdd(queue, chunk);
+                g = __shfl(g, 0, 64);
+                if (g >= n) {
+                    exhausted = true;
+                } else {
+                    pool_next = g;
+                    pool_end = min(g + chunk, n);
+                }
+            }
+            uint32_t avail = pool_end - pool_next;
+            if (!has) {
+                uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32),
+                                                          __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
+                if (rank < avail) {
+                    idx = pool_next + rank;
+                    ray_setup(st, r.ox[idx], r.oy[idx], r.oz[idx], r.dx[idx], r.dy[idx], r.dz[idx], r.W[0], r.W[1],
+                              r.W[2]);
+                    has = true;
+                }
+            }
+            pool_next += min(avail, nidle);
+        }
+        if (!__any(has)) {
+            if (exhausted || pool_next == pool_end) {
+                // nothing in flight: either all work is done or the pool is empty and
+                // the next iteration grabs a new chunk
+                if (exhausted) break;
+                continue;
+            }
+        }
+        if (has) {
+            int32_t v[3];
+            uint32_t slot = 0;
+            int res = st.iters < max_iters ? ray_step(w, st, v, &slot) : STEP_MISS;
+            if (res != STEP_CONTINUE) {
+                float t;
+                int4 rec;
+                if (res == STEP_HIT)
+                    ray_hit_record(w, st, v, slot, t, rec);
+                else
+                    ray_miss_record(st, t, rec);
+                store_result(h, idx, st, t, rec);
+                if constexpr (COUNT) {
+                    c_rays++;
+                    c_iters += st.iters;
+                    c_desc += st.descents;
+                    c_hits += (rec.w & 0xFF) != 0;
+                }
+                has = false;
+            }
+        }
+    }
+    if constexpr (COUNT) flush_counters(ctr, c_rays, c_iters, c_desc, c_hits);
+}
+
+__global__ void fill_u32_kernel(uint32_t* p, uint32_t n, uint32_t v) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------- device properties and queues
+
+struct DeviceInfo {
+    int cus = 256;
+    uint32_t* queues = nullptr;  // ring of queue counters for persistent launches
+    std::atomic<uint32_t> next{0};
+};
+static DeviceInfo g_dev[64];
+static std::mutex g_dev_mu;
+
+static const uint32_t kQueueRing = 4096;
+
+int device_info(int dev, int* cus, uint32_t** queue) {
+    if (dev < 0 || dev >= 64) return fail(VRT_ERR_NO_DEVICE, "device index out of range");
+    DeviceInfo& di = g_dev[dev];
+    {
+        std::lock_guard<std::mutex> lk(g_dev_mu);
+        if (!di.queues) {
+            hipDeviceProp_t prop;
+            VRT_HIP(hipGetDeviceProperties(&prop, dev));
+            di.cus = prop.multiProcessorCount;
+            VRT_HIP(hipMalloc(&di.queues, kQueueRing * sizeof(uint32_t)));
+        }
+    }
+    *cus = di.cus;
+    if (queue) *queue = di.queues + (di.next.fetch_add(1) % kQueueRing);
+    return VRT_OK;
+}
+
+// ---------------------------------------------------------------- launch
+
+int launch_cast(const vrt_world* world, const RaysIn& r, const HitsOut& h, uint32_t max_iters,
+                const vrt_cast_opts& o, hipStream_t s) {
+    WorldView w = world->view();
+    if (r.n == 0) return VRT_OK;
+    uint32_t sched = o.schedule == VRT_SCHED_AUTO ? VRT_SCHED_GRID : o.schedule;
+    if (sched == VRT_SCHED_GRID) {
+        uint32_t blocks = (r.n + 255) / 256;
+        if (o.counters)
+            cast_grid_kernel<true><<<blocks, 256, 0, s>>>(w, r, h, max_iters, o.counters);
+        else
+            cast_grid_kernel<false><<<blocks, 256, 0, s>>>(w, r, h, max_iters, nullptr);
+        VRT_HIP(hipGetLastError());
+        return VRT_OK;
+    }
+    if (sched != VRT_SCHED_PERSISTENT) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_cast_ex: unknown schedule");
+    uint32_t chunk = o.chunk ? o.chunk : 256;
+    if (chunk % 64 != 0 || chunk > (1u << 20)) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_cast_ex: chunk must be a multiple of 64");
+    int cus = 0;
+    uint32_t* queue = nullptr;
+    int rc = device_info(world->device, &cus, &queue);
+    if (rc != VRT_OK) return rc;
+    uint32_t wpc = o.waves_per_cu ? o.waves_per_cu : 32;
+    uint32_t blocks = std::max<uint32_t>(1, (uint32_t)cus * wpc / 4);
+    uint32_t max_blocks = (r.n + 255) / 256;
+    blocks = std::min(blocks, std::max<uint32_t>(1, max_blocks));
+    uint32_t refill_min = o.rays_per_lane >= 64 ? 64 : 16;
+    fill_u32_kernel<<<1, 64, 0, s>>>(queue, 1, 0u);
+    if (o.counters)
+        cast_persistent_kernel<true><<<blocks, 256, 0, s>>>(w, r, h, max_iters, chunk, refill_min, queue, o.counters);
+    else
+        cast_persistent_kernel<false><<<blocks, 256, 0, s>>>(w, r, h, max_iters, chunk, refill_min, queue, nullptr);
+    VRT_HIP(hipGetLastError());
+    return VRT_OK;
+}
+
+int check_rays_hits(const vrt_rays* rays, const vrt_hits* hits, RaysIn& r, HitsOut& h) {
+    if (!rays || !hits) return fail(VRT_ERR_INVALID_ARGUMENT, "null rays or hits");
+    if (rays->n && (!rays->ox || !rays->oy || !rays->oz || !rays->dx || !rays->dy || !rays->dz))
+        return fail(VRT_ERR_INVALID_ARGUMENT, "null ray buffer");
+    if (rays->n && (!hits->t || !hits->rec)) return fail(VRT_ERR_INVALID_ARGUMENT, "null hit buffer");
+    if (((uintptr_t)hits->rec & 15) != 0) return fail(VRT_ERR_INVALID_ARGUMENT, "hit records must be 16-byte aligned");
+    r.ox = rays->ox; r.oy = rays->oy; r.oz = rays->oz;
+    r.dx = rays->dx; r.dy = rays->dy; r.dz = rays->dz;
+    r.W[0] = rays->world_origin[0]; r.W[1] = rays->world_origin[1]; r.W[2] = rays->world_origin[2];
+    r.n = rays->n;
+    h.t = hits->t;
+    h.rec = (int4*)hits->rec;
+    if (hits->pos) {
+        h.pos_x = hits->pos;
+        h.pos_y = hits->pos + rays->n;
+        h.pos_z = hits->pos + 2 * (size_t)rays->n;
+    } else {
+        h.pos_x = h.pos_y = h.pos_z = nullptr;
+    }
+    return VRT_OK;
+}
+
+// ---------------------------------------------------------------- FP64 picking cast
+
+// VoxelMap::RayCast (VoxelMap.cpp:140-170) with GetStepLevel (:125-139): a
+// single ray in FP64, bias 1e-4, fixed step levels 32 / 8 / 1 voxels.  Sector
+// coordinates wrap like WorldSectorIndexer<12,8,signed> (VoxelMap.h:100).
+__device__ __forceinline__ int32_t d2i_floor(double x) {
+    double f = floor(x);
+    return fabs(f) < 2147483648.0 ? (int32_t)f : INT32_MIN;
+}
+
+__global__ void cast_one_f64_kernel(WorldView w, double ox, double oy, double oz, double dx, double dy, double dz,
+                                    uint32_t max_iters, vrt_hit_f64* out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double o[3] = {ox, oy, oz}, d[3] = {dx, dy, dz}, inv[3], ts[3];
+    int32_t p[3];
+    for (int a = 0; a < 3; a++) {
+        inv[a] = 1.0 / d[a];
+        ts[a] = ((d[a] < 0.0 ? 0.0 : 1.0) - o[a]) * inv[a];
+        p[a] = d2i_floor(o[a]);
+    }
+    vrt_hit_f64 res;
+    res.distance = -1.0;
+    for (int a = 0; a < 3; a++) { res.normal[a] = 0.0f; res.voxel[a] = 0; }
+    res.uv[0] = res.uv[1] = 0.0f;
+    uint32_t nxz = 1u << w.shift_xz, ny = 1u << w.shift_y;
+    for (uint32_t i = 0; i < max_iters; i++) {
+        double side[3], hp[3];
+        for (int a = 0; a < 3; a++) side[a] = fma((double)p[a], inv[a], ts[a]);
+        double tmin = side[0];
+        tmin = side[1] < tmin ? side[1] : tmin;  // glm::min(a, b) = b < a ? b : a
+        tmin = side[2] < tmin ? side[2] : tmin;
+        tmin += 0.0001;
+        for (int a = 0; a < 3; a++) {
+            hp[a] = fma(tmin, d[a], o[a]);
+            p[a] = d2i_floor(hp[a]);
+        }
+        // GetStepLevel
+        int k;
+        uint32_t sx = (uint32_t)(p[0] >> 5) & 4095u, sy = (uint32_t)(p[1] >> 5) & 255u, sz = (uint32_t)(p[2] >> 5) & 4095u;
+        uint64_t m = 0;
+        uint32_t si = 0;
+        if (sx < nxz && sz < nxz && sy < ny) {
+            si = sx | sz << w.shift_xz | sy << (2 * w.shift_xz);
+            m = w.mask[si];
+        }
+        if (m == 0) {
+            k = 5;
+        } else {
+            uint32_t bi = ((uint32_t)(p[0] >> 3) & 3u) | ((uint32_t)(p[2] >> 3) & 3u) << 2 | ((uint32_t)(p[1] >> 3) & 3u) << 4;
+            if (!((m >> bi) & 1ull)) {
+                k = 3;
+            } else {
+                uint32_t slot = w.base[si] + (uint32_t)__popcll(m & ((1ull << bi) - 1ull));
+                uint32_t vi = ((uint32_t)p[0] & 7u) | ((uint32_t)p[2] & 7u) << 3 | ((uint32_t)p[1] & 7u) << 6;
+                k = w.vox[(size_t)slot * 512 + vi] == 0 ? 0 : -1;
+            }
+        }
+        if (k < 0) {
+            bool sm[3];
+            for (int a = 0; a < 3; a++) {
+                sm[a] = tmin >= side[a];
+                double sg = d[a] > 0 ? 1.0 : (d[a] < 0 ? -1.0 : 0.0);
+                res.normal[a] = sm[a] ? (float)(-sg) : 0.0f;
+                res.voxel[a] = p[a];
+            }
+            float u = sm[0] ? (float)hp[1] : (float)hp[0];
+            float v = sm[2] ? (float)hp[1] : (float)hp[2];
+            res.uv[0] = u - floorf(u);
+            res.uv[1] = v - floorf(v);
+            res.distance = tmin;
+            break;
+        }
+        int32_t mk = (int32_t)((1u << k) - 1u);
+        for (int a = 0; a < 3; a++) p[a] = d[a] < 0 ? (p[a] & ~mk) : (p[a] | mk);
+    }
+    *out = res;
+}
+
+}  // namespace vrt
+
+using namespace vrt;
+
+extern "C" {
+
+int vrt_cast(const vrt_world* world, const vrt_rays* rays, uint32_t max_iters, vrt_hits* hits, void* stream) {
+    vrt_cast_opts o;
+    std::memset(&o, 0, sizeof(o));
+    return vrt_cast_ex(world, rays, max_iters, hits, &o, stream);
+}
+
+int vrt_cast_ex(const vrt_world* world, const vrt_rays* rays, uint32_t max_iters, vrt_hits* hits,
+                const vrt_cast_opts* opts, void* stream) {
+    return guarded([&]() -> int {
+        if (!world) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_cast: null world");
+        if (max_iters == 0 || max_iters > 65535) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_cast: max_iters must be 1..65535");
+        if (fault_injected("vrt_cast")) return fail(VRT_ERR_HIP, "vrt_cast: injected HIP fault");
+        RaysIn r;
+        HitsOut h;
+        int rc = check_rays_hits(rays, hits, r, h);
+        if (rc != VRT_OK) return rc;
+        vrt_cast_opts o;
+        std::memset(&o, 0, sizeof(o));
+        if (opts) o = *opts;
+        return launch_cast(world, r, h, max_iters, o, (hipStream_t)stream);
+    });
+}
+
+size_t vrt_cast_scratch_bytes(void) { return kQueueRing * sizeof(uint32_t); }
+
+int vrt_cast_one_f64(const vrt_world* world, const double origin[3], const double dir[3], uint32_t max_iters,
+                     vrt_hit_f64* out) {
+    return guarded([&]() -> int {
+        if (!world || !origin || !dir || !out) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_cast_one_f64: null argument");
+        if (world->log2_y > 13) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_cast_one_f64: world taller than 8192 voxels");
+        vrt_hit_f64* d_out = nullptr;
+        VRT_HIP(hipMalloc(&d_out, sizeof(vrt_hit_f64)));
+        cast_one_f64_kernel<<<1, 64, 0, 0>>>(world->view(), origin[0], origin[1], origin[2], dir[0], dir[1], dir[2],
+                                             max_iters, d_out);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpy(out, d_out, sizeof(vrt_hit_f64), hipMemcpyDeviceToHost);
+        (void)hipFree(d_out);
+        if (e != hipSuccess) return hip_fail(e, "vrt_cast_one_f64");
+        return VRT_OK;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,163 @@
+// rays.hip - primary-ray generation and diffuse bounce generation on the device.
+//
+// vrt_camera_rays replaces GetPrimaryRay (src/VoxelRT/CpuRenderer.cpp:226-233,
+// Shaders/VoxelRender.comp:20-25) with the pinhole camera of vrt_math.h.
+// vrt_bounce_generate replaces the bounce construction of RenderRow
+// (CpuRenderer.cpp:389-392) / VoxelRender.comp:63-71, with the PCG sample of
+// RandomGen.glsl:6-20, and compacts the bounce rays in primary order
+// (count -> scan -> scatter; deterministic, no atomics).
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+
+#include "vrt_cast_impl.h"
+#include "vrt_math.h"
+
+namespace vrt {
+
+__global__ __launch_bounds__(256) void camera_rays_kernel(vrt_camera cam, float inv_w, float inv_h, uint32_t n,
+                                                          float* __restrict__ ox, float* __restrict__ oy,
+                                                          float* __restrict__ oz, float* __restrict__ dx,
+                                                          float* __restrict__ dy, float* __restrict__ dz) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t px, py;
+    tile_pixel(i, cam.width, cam.tile_w, cam.tile_h, px, py);
+    float d[3];
+    camera_ray(cam.forward, cam.right, cam.up, inv_w, inv_h, px, py, d);
+    ox[i] = cam.eye[0];
+    oy[i] = cam.eye[1];
+    oz[i] = cam.eye[2];
+    dx[i] = d[0];
+    dy[i] = d[1];
+    dz[i] = d[2];
+}
+
+static const uint32_t kBounceBlock = 256;
+
+// Number of hits in each block of kBounceBlock primaries.
+__global__ __launch_bounds__(256) void bounce_count_kernel(const int4* __restrict__ rec, uint32_t n,
+                                                           uint32_t* __restrict__ block_counts) {
+    __shared__ uint32_t wave_counts[4];
+    uint32_t i = blockIdx.x * kBounceBlock + threadIdx.x;
+    bool hit = i < n && (rec[i].w & 0xFF) != 0;
+    uint64_t b = __ballot(hit);
+    if (__lane_id() == 0) wave_counts[threadIdx.x >> 6] = (uint32_t)__popcll(b);
+    __syncthreads();
+    if (threadIdx.x == 0) block_counts[blockIdx.x] = wave_counts[0] + wave_counts[1] + wave_counts[2] + wave_counts[3];
+}
+
+// Exclusive scan of the block counts in one workgroup; writes the total to *count.
+__global__ __launch_bounds__(1024) void bounce_scan_kernel(uint32_t* __restrict__ block_counts, uint32_t nblocks,
+                                                           uint32_t* __restrict__ count) {
+    __shared__ uint32_t partial[1024];
+    uint32_t per = (nblocks + 1023) / 1024;
+    uint32_t begin = threadIdx.x * per, end = min(begin + per, nblocks);
+    uint32_t sum = 0;
+    for (uint32_t k = begin; k < end; k++) sum += block_counts[k];
+    partial[threadIdx.x] = sum;
+    __syncthreads();
+    // Hillis-Steele inclusive scan over the 1024 partial sums
+    for (uint32_t off = 1; off < 1024; off <<= 1) {
+        uint32_t v = threadIdx.x >= off ? partial[threadIdx.x - off] : 0u;
+        __syncthreads();
+        partial[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = threadIdx.x ? partial[threadIdx.x - 1] : 0u;
+    for (uint32_t k = begin; k < end; k++) {
+        uint32_t c = block_counts[k];
+        block_counts[k] = run;
+        run += c;
+    }
+    if (threadIdx.x == 1023) *count = partial[1023];
+}
+
+__global__ __launch_bounds__(256) void bounce_write_kernel(const int4* __restrict__ rec, const float* __restrict__ px,
+                                                           const float* __restrict__ py, const float* __restrict__ pz,
+                                                           uint32_t n, uint32_t width, uint32_t seed,
+                                                           const uint32_t* __restrict__ block_offsets,
+                                                           vrt_ray_buffers out, uint32_t* __restrict__ src) {
+    __shared__ uint32_t wave_counts[4];
+    uint32_t i = blockIdx.x * kBounceBlock + threadIdx.x;
+    int4 r = i < n ? rec[i] : make_int4(0, 0, 0, 0);
+    bool hit = i < n && (r.w & 0xFF) != 0;
+    uint64_t b = __ballot(hit);
+    uint32_t wave = threadIdx.x >> 6;
+    if (__lane_id() == 0) wave_counts[wave] = (uint32_t)__popcll(b);
+    __syncthreads();
+    if (!hit) return;
+    uint32_t base = block_offsets[blockIdx.x];
+    for (uint32_t w = 0; w < wave; w++) base += wave_counts[w];
+    uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(b >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u));
+    uint32_t k = base + rank;
+    float p[3] = {px[i], py[i], pz[i]};
+    float o[3], d[3];
+    bounce_ray((uint32_t)r.w, p, i, width, seed, o, d);
+    out.ox[k] = o[0];
+    out.oy[k] = o[1];
+    out.oz[k] = o[2];
+    out.dx[k] = d[0];
+    out.dy[k] = d[1];
+    out.dz[k] = d[2];
+    src[k] = i;
+}
+
+}  // namespace vrt
+
+using namespace vrt;
+
+extern "C" {
+
+int vrt_camera_rays(const vrt_camera* cam, vrt_ray_buffers* out, void* stream) {
+    return guarded([&]() -> int {
+        if (!cam || !out) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_camera_rays: null argument");
+        if (!cam->tile_w || !cam->tile_h || !cam->width || !cam->height || cam->width % cam->tile_w ||
+            cam->height % cam->tile_h)
+            return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_camera_rays: size must be a positive multiple of the tile");
+        if (!out->ox || !out->oy || !out->oz || !out->dx || !out->dy || !out->dz)
+            return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_camera_rays: null ray buffer");
+        uint64_t n64 = (uint64_t)cam->width * cam->height;
+        if (n64 > 0xFFFFFFFFull) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_camera_rays: too many pixels");
+        uint32_t n = (uint32_t)n64;
+        camera_rays_kernel<<<(n + 255) / 256, 256, 0, (hipStream_t)stream>>>(
+            *cam, 1.0f / (float)cam->width, 1.0f / (float)cam->height, n, out->ox, out->oy, out->oz, out->dx, out->dy,
+            out->dz);
+        VRT_HIP(hipGetLastError());
+        return VRT_OK;
+    });
+}
+
+size_t vrt_bounce_scratch_bytes(uint32_t n) {
+    return (size_t)((n + kBounceBlock - 1) / kBounceBlock + 1) * sizeof(uint32_t);
+}
+
+int vrt_bounce_generate(const vrt_rays* primary, const vrt_hits* hits, uint32_t width, uint32_t seed,
+                        vrt_ray_buffers* out, uint32_t* src, uint32_t* count, void* scratch, void* stream) {
+    return guarded([&]() -> int {
+        if (!primary || !hits || !out || !src || !count || !scratch)
+            return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_bounce_generate: null argument");
+        if (!hits->pos) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_bounce_generate: hits need positions");
+        if (width == 0) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_bounce_generate: width must be > 0");
+        if (!out->ox || !out->oy || !out->oz || !out->dx || !out->dy || !out->dz)
+            return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_bounce_generate: null ray buffer");
+        hipStream_t s = (hipStream_t)stream;
+        uint32_t n = primary->n;
+        uint32_t nb = (n + kBounceBlock - 1) / kBounceBlock;
+        uint32_t* blocks = (uint32_t*)scratch;
+        if (n == 0) {
+            hipLaunchKernelGGL(bounce_scan_kernel, dim3(1), dim3(1024), 0, s, blocks, 0u, count);
+            VRT_HIP(hipGetLastError());
+            return VRT_OK;
+        }
+        const int4* rec = (const int4*)hits->rec;
+        bounce_count_kernel<<<nb, 256, 0, s>>>(rec, n, blocks);
+        bounce_scan_kernel<<<1, 1024, 0, s>>>(blocks, nb, count);
+        bounce_write_kernel<<<nb, 256, 0, s>>>(rec, hits->pos, hits->pos + n, hits->pos + 2 * (size_t)n, n, width,
+                                               seed, blocks, *out, src);
+        VRT_HIP(hipGetLastError());
+        return VRT_OK;
+    });
+}
+
+}  // extern "C"

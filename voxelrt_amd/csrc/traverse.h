@@ -1,0 +1,105 @@
+// traverse.h - the per-ray brickmap DDA on the device.
+//
+// One lane owns one ray.  The arithmetic is the reference CPU cast
+// (src/VoxelRT/CpuRenderer.cpp:135-224) per lane: FP32, IEEE 1/d, explicit
+// fmaf for sideDist and the next position, x86 min and floor2i semantics,
+// +0.001 bias, 2-level LOD from the 64-bit masks (brick level 8/16/32 voxels,
+// voxel level 1/2/4).  Storage is the popcount-compacted GPU layout
+// (GpuRenderer.cpp:27-33, VoxelMap.glsl:45-64) instead of the CPU's dense one.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "vrt_internal.h"
+#include "vrt_math.h"
+
+namespace vrt {
+
+struct RayState {
+    float o[3], d[3], inv[3], ts[3], side[3], cur[3];
+    int32_t W[3];
+    uint32_t iters, descents;
+};
+
+enum StepResult : int { STEP_CONTINUE = 0, STEP_HIT = 1, STEP_MISS = 2 };
+
+__device__ __forceinline__ void ray_setup(RayState& r, float ox, float oy, float oz, float dx, float dy, float dz,
+                                          int32_t wx, int32_t wy, int32_t wz) {
+    r.o[0] = ox; r.o[1] = oy; r.o[2] = oz;
+    r.d[0] = dx; r.d[1] = dy; r.d[2] = dz;
+    r.W[0] = wx; r.W[1] = wy; r.W[2] = wz;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        r.inv[a] = 1.0f / r.d[a];
+        r.ts[a] = ((r.d[a] < 0.0f ? 0.0f : 1.0f) - r.o[a]) * r.inv[a];
+        r.side[a] = 0.0f;
+        r.cur[a] = r.o[a];
+    }
+    r.iters = 0;
+    r.descents = 0;
+}
+
+// One iteration of RayCast (CpuRenderer.cpp:185-202) with GetStepPos (:135-171).
+// On STEP_HIT, v is the hit voxel and *hit_slot its brick slot.
+__device__ __forceinline__ int ray_step(const WorldView& w, RayState& r, int32_t v[3], uint32_t* hit_slot) {
+#pragma unroll
+    for (int a = 0; a < 3; a++) v[a] = (int32_t)((uint32_t)r.W[a] + (uint32_t)floor2i(r.cur[a]));
+    // GetInboundMask (:114-117)
+    if (!((uint32_t)(v[0] | v[2]) < w.size_xz && (uint32_t)v[1] < w.size_y)) return STEP_MISS;
+    r.iters++;
+    uint32_t ux = (uint32_t)v[0], uy = (uint32_t)v[1], uz = (uint32_t)v[2];
+    uint32_t si = (ux >> 5) | (uz >> 5) << w.shift_xz | (uy >> 5) << (2 * w.shift_xz);
+    uint64_t m = w.mask[si];
+    uint32_t bi = ((ux >> 3) & 3u) | ((uz >> 3) & 3u) << 2 | ((uy >> 3) & 3u) << 4;
+    uint32_t half = bi < 32 ? (uint32_t)m : (uint32_t)(m >> 32);
+    uint32_t lod = 3;
+    if ((half >> (bi & 31u)) & 1u) {
+        r.descents++;
+        uint32_t slot = w.base[si] + (uint32_t)__popcll(m & ((1ull << bi) - 1ull));
+        uint32_t cell = ((ux >> 2) & 1u) | ((uz >> 2) & 1u) << 1 | ((uy >> 2) & 1u) << 2;
+        m = w.occ[(size_t)slot * 8 + cell];
+        bi = (ux & 3u) | (uz & 3u) << 2 | (uy & 3u) << 4;
+        half = bi < 32 ? (uint32_t)m : (uint32_t)(m >> 32);
+        lod = 0;
+        if ((half >> (bi & 31u)) & 1u) {
+            *hit_slot = slot;
+            return STEP_HIT;
+        }
+    }
+    // level4: the 4x4x4 mask is empty; level2: the 2x2x2 group of bi is empty
+    if (m == 0)
+        lod += 2;
+    else if (((half >> (bi & 0xAu)) & 0x00330033u) == 0)
+        lod += 1;
+    int32_t cm = (int32_t)((1u << lod) - 1u);
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        v[a] = r.d[a] < 0.0f ? (v[a] & ~cm) : (v[a] | cm);
+        int32_t rel = (int32_t)((uint32_t)v[a] - (uint32_t)r.W[a]);
+        r.side[a] = fmaf((float)rel, r.inv[a], r.ts[a]);
+    }
+    float tmin = min_x86(min_x86(r.side[0], r.side[1]), r.side[2]) + 0.001f;
+#pragma unroll
+    for (int a = 0; a < 3; a++) r.cur[a] = fmaf(tmin, r.d[a], r.o[a]);
+    return STEP_CONTINUE;
+}
+
+// Hit record fields (vrt_hit in vrt.h) of a finished ray.
+__device__ __forceinline__ void ray_hit_record(const WorldView& w, const RayState& r, const int32_t v[3],
+                                               uint32_t slot, float& t, int4& rec) {
+    float tt = min_x86(min_x86(r.side[0], r.side[1]), r.side[2]);
+    bool X = r.side[0] == tt, Y = r.side[1] == tt, Z = !X && !Y;
+    uint32_t code = (X ? (signbit(r.d[0]) ? 1u : 3u) : 0u) | (Y ? (signbit(r.d[1]) ? 1u : 3u) : 0u) << 2 |
+                    (Z ? (signbit(r.d[2]) ? 1u : 3u) : 0u) << 4;
+    uint32_t ux = (uint32_t)v[0], uy = (uint32_t)v[1], uz = (uint32_t)v[2];
+    uint32_t mat = w.vox[(size_t)slot * 512 + ((ux & 7u) | (uz & 7u) << 3 | (uy & 7u) << 6)];
+    t = tt;
+    rec = make_int4(v[0], v[1], v[2], (int)(mat | code << 8 | (r.iters & 0xFFFFu) << 16));
+}
+
+__device__ __forceinline__ void ray_miss_record(const RayState& r, float& t, int4& rec) {
+    t = -1.0f;
+    rec = make_int4(0, 0, 0, (int)((r.iters & 0xFFFFu) << 16));
+}
+
+}  // namespace vrt

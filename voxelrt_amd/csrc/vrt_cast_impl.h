@@ -1,0 +1,38 @@
+// vrt_cast_impl.h - kernel-side ray/hit views shared by the cast, bounce,
+// batched and differentiable translation units (not part of the ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "vrt_internal.h"
+
+namespace vrt {
+
+struct RaysIn {
+    const float* __restrict__ ox;
+    const float* __restrict__ oy;
+    const float* __restrict__ oz;
+    const float* __restrict__ dx;
+    const float* __restrict__ dy;
+    const float* __restrict__ dz;
+    int32_t W[3];
+    uint32_t n;
+};
+
+struct HitsOut {
+    float* __restrict__ t;
+    int4* __restrict__ rec;
+    float* __restrict__ pos_x;  // all three null when positions are not requested
+    float* __restrict__ pos_y;
+    float* __restrict__ pos_z;
+};
+
+// Validates the ABI structs and converts them to kernel views.
+int check_rays_hits(const vrt_rays* rays, const vrt_hits* hits, RaysIn& r, HitsOut& h);
+// Launches a cast with the given options on stream s.
+int launch_cast(const vrt_world* world, const RaysIn& r, const HitsOut& h, uint32_t max_iters,
+                const vrt_cast_opts& o, hipStream_t s);
+// Compute units of device `dev` and, when queue is not null, a queue counter from the device's ring.
+int device_info(int dev, int* cus, uint32_t** queue);
+
+}  // namespace vrt

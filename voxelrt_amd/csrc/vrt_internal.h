@@ -1,0 +1,119 @@
+// vrt_internal.h - shared internals of libvoxelrt_hip.so (not part of the ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "vrt.h"
+
+namespace vrt {
+
+// ---------------------------------------------------------------- errors
+
+void set_error(const std::string& msg);
+void clear_error();
+
+// Returns a vrt_status after recording the message.
+int fail(int status, const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+
+#define VRT_HIP(call)                                     \
+    do {                                                  \
+        hipError_t e_ = (call);                           \
+        if (e_ != hipSuccess) return ::vrt::hip_fail(e_, #call); \
+    } while (0)
+
+// Wraps a body so that no C++ exception crosses the ABI.
+template <typename F>
+int guarded(F&& f) {
+    try {
+        clear_error();
+        return f();
+    } catch (const std::bad_alloc&) {
+        return fail(VRT_ERR_CAPACITY, "host allocation failed");
+    } catch (const std::exception& ex) {
+        return fail(VRT_ERR_STATE, ex.what());
+    } catch (...) {
+        return fail(VRT_ERR_STATE, "unknown exception");
+    }
+}
+
+// Test hook: VRT_FAULT_INJECT=<entry name> makes that entry report a HIP error.
+bool fault_injected(const char* entry);
+
+// ---------------------------------------------------------------- slot arena
+
+// Free-list arena of brick slots.  Each sector owns one contiguous run of
+// popcount(mask) slots.  Slot 0 is valid (no sentinel).  Best-fit allocation,
+// growth in place when the following range is free, coalescing on free.
+class SlotArena {
+public:
+    explicit SlotArena(uint32_t capacity = 0) : capacity_(capacity) {
+        if (capacity) free_[0] = capacity;
+    }
+    // Moves/grows a run from (base, cur) to a run of `want` slots. Returns false when full.
+    bool realloc(uint32_t& base, uint32_t cur, uint32_t want);
+    void release(uint32_t base, uint32_t size);
+    uint32_t allocated() const { return allocated_; }
+    uint32_t capacity() const { return capacity_; }
+    size_t free_ranges() const { return free_.size(); }
+    // One past the highest allocated slot.
+    uint32_t high_water() const;
+
+private:
+    std::map<uint32_t, uint32_t> free_;  // start -> length
+    uint32_t capacity_ = 0;
+    uint32_t allocated_ = 0;
+};
+
+// ---------------------------------------------------------------- device world view
+
+struct WorldView {
+    const uint64_t* __restrict__ mask;   // [nsec]
+    const uint32_t* __restrict__ base;   // [nsec]
+    const uint64_t* __restrict__ occ;    // [slots * 8]
+    const uint8_t* __restrict__ vox;     // [slots * 512]
+    uint32_t size_xz, size_y;            // voxel extents
+    uint32_t shift_xz, shift_y;          // log2 sector grid extents
+};
+
+}  // namespace vrt
+
+struct vrt_world {
+    int device = 0;
+    uint32_t log2_xz = 0, log2_y = 0;
+    uint32_t shift_xz = 0, shift_y = 0;
+    uint32_t nsec = 0;
+    uint64_t* d_mask = nullptr;
+    uint32_t* d_base = nullptr;
+    uint64_t* d_occ = nullptr;
+    uint8_t* d_vox = nullptr;
+    uint64_t* d_palette = nullptr;
+    uint32_t slot_cap = 0;            // device slots backed by memory
+    std::vector<uint64_t> h_mask;     // host mirror of the sector headers
+    std::vector<uint32_t> h_base;
+    vrt::SlotArena arena;
+    uint32_t num_bricks = 0;
+
+    vrt::WorldView view() const {
+        vrt::WorldView v;
+        v.mask = d_mask;
+        v.base = d_base;
+        v.occ = d_occ;
+        v.vox = d_vox;
+        v.size_xz = 1u << log2_xz;
+        v.size_y = 1u << log2_y;
+        v.shift_xz = shift_xz;
+        v.shift_y = shift_y;
+        return v;
+    }
+};
+
+namespace vrt {
+// Launches the occupancy rebuild of `nslots` slots listed in d_slots (device) on stream.
+hipError_t launch_build_occupancy(vrt_world* w, const uint32_t* d_slots, uint32_t nslots, hipStream_t s);
+}  // namespace vrt
