@@ -1,0 +1,269 @@
+#!/usr/bin/env python3
+"""Benchmark: brickmap ray traversal on one MI355X.
+
+Workload (one "step"): the 1024^3 terrain + caves world (seeded fBm heightmap,
+3D value-noise caves, ~12% of bricks allocated), 1920x1080 primary rays from a
+fixed camera above the terrain, then one cosine-distributed diffuse bounce ray
+per primary hit (seeded PCG), i.e.
+    vrt_cast(primary) -> vrt_bounce_generate -> vrt_cast(bounce, device count)
+Rays, world and output buffers are resident in HBM before timing starts.
+value = rays traced (primary + bounce) per second, in millions.
+
+Also reported: the HBM roofline of the cast kernel (algorithmic bytes per
+launch from the kernel's own counters / average launch time from HIP events
+on the launch stream) and the reference CPU path (the oracle's SIMD port of
+CpuRenderer.cpp:172-224) timed on this host's cores on the same ray buffers.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--tile 8x8] [--schedule grid|persistent]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes: ray in, hit out, per-lookup sector mask,
+# per-descent base slot + occupancy mask, per-hit voxel byte
+B_RAY_IN = 24
+B_HIT_OUT = 20
+B_POS_OUT = 12
+B_ITER = 8
+B_DESCENT = 12
+B_HIT = 1
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(c, with_pos: bool) -> float:
+    rays, iters, desc, hits = (float(x) for x in c)
+    return rays * (B_RAY_IN + B_HIT_OUT + (B_POS_OUT if with_pos else 0)) + iters * B_ITER + desc * B_DESCENT + hits * B_HIT
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--tile", default="8x8")
+    ap.add_argument("--schedule", default="grid", choices=["grid", "persistent"])
+    ap.add_argument("--chunk", type=int, default=0)
+    ap.add_argument("--cpu-runs", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="also run one device copy of this many MiB (a known-size kernel for scaling PMC byte counters)")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (rays shard across GPUs "
+              "with a replicated world, see DESIGN.md)", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    import voxelrt_amd as V
+    from voxelrt_amd import workloads as W
+
+    tile = tuple(int(x) for x in args.tile.split("x"))
+    sched = 1 if args.schedule == "grid" else 2
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    stream = torch.cuda.current_stream()
+
+    log("building the 1024^3 terrain + caves world")
+    t0 = time.time()
+    scene = W.C2_WORLD.build()
+    log(f"  generated in {time.time() - t0:.1f} s, brick fraction {W.brick_fraction(scene):.4f}")
+    world = V.World.from_scene(scene)
+    info = world.info()
+    cam = W.c2_camera(tile=tile)
+    prim = cam.rays_device()
+    n = prim.n
+    hits_p = V.HitBatch.empty(n, with_pos=True)
+    hits_b = V.HitBatch.empty(n, with_pos=False)
+    bounce_rays = V.RayBatch.empty(n, prim.world_origin)
+    src = torch.empty(n, dtype=torch.int32, device=dev)
+    count = torch.zeros(1, dtype=torch.int32, device=dev)
+    scratch = torch.empty(int(V.lib().vrt_bounce_scratch_bytes(n)) // 4 + 1, dtype=torch.int32, device=dev)
+
+    import ctypes as C
+    from voxelrt_amd import _lib as L
+
+    lib = V.lib()
+    r_prim = prim.to_c()
+    h_prim = hits_p.to_c()
+    r_b = bounce_rays.to_c()
+    h_b = hits_b.to_c()
+    bufs_b = bounce_rays.buffers()
+    sh = stream.cuda_stream
+
+    def opts(counters=None, cnt=None):
+        return L.CastOpts(sched, 1, args.chunk, 0, counters.data_ptr() if counters is not None else 0,
+                          cnt.data_ptr() if cnt is not None else 0)
+
+    o_plain = opts()
+    o_count = opts(cnt=count)
+
+    def cast_primary(o=o_plain):
+        L.check(lib.vrt_cast_ex(world.handle, C.byref(r_prim), W.MAX_ITERS, C.byref(h_prim), C.byref(o), sh))
+
+    def gen_bounce():
+        L.check(lib.vrt_bounce_generate(C.byref(r_prim), C.byref(h_prim), cam.c.width, W.BOUNCE_SEED,
+                                        C.byref(bufs_b), src.data_ptr(), count.data_ptr(), scratch.data_ptr(), sh))
+
+    def cast_bounce(o=o_count):
+        L.check(lib.vrt_cast_ex(world.handle, C.byref(r_b), W.MAX_ITERS, C.byref(h_b), C.byref(o), sh))
+
+    if args.copy_mb:
+        a = torch.empty(args.copy_mb << 18, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        a.fill_(1.0)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+
+    for _ in range(args.warmup):
+        cast_primary()
+        gen_bounce()
+        cast_bounce()
+    torch.cuda.synchronize()
+    nb = int(count.item())
+
+    # counters (same traversal, counting kernel variant; outside the timed region)
+    cp = torch.zeros(4, dtype=torch.int64, device=dev)
+    cb = torch.zeros(4, dtype=torch.int64, device=dev)
+    cast_primary(opts(counters=cp))
+    gen_bounce()
+    cast_bounce(opts(counters=cb, cnt=count))
+    torch.cuda.synchronize()
+    cp_h, cb_h = cp.cpu().numpy(), cb.cpu().numpy()
+    assert int(cp_h[0]) == n and int(cb_h[0]) == nb, (cp_h, cb_h, n, nb)
+
+    # timed region: K steps, HIP events around every cast launch on the launch stream
+    K = args.steps
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(K)]
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for k in range(K):
+        ev[k][0].record(stream)
+        cast_primary()
+        ev[k][1].record(stream)
+        gen_bounce()
+        ev[k][2].record(stream)
+        cast_bounce()
+        ev[k][3].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_start
+    step_ms = [ev[k][0].elapsed_time(ev[k][3]) for k in range(K)]
+    prim_ms = [ev[k][0].elapsed_time(ev[k][1]) for k in range(K)]
+    bounce_ms = [ev[k][2].elapsed_time(ev[k][3]) for k in range(K)]
+    gen_ms = [ev[k][1].elapsed_time(ev[k][2]) for k in range(K)]
+    gpu_s = sum(step_ms) / 1e3
+    rays_per_step = n + nb
+    value = rays_per_step * K / wall / 1e6
+    ms_per_step = wall * 1e3 / K
+
+    bytes_p = algorithmic_bytes(cp_h, True)
+    bytes_b = algorithmic_bytes(cb_h, False)
+    avg_launch_s = (statistics.mean(prim_ms) + statistics.mean(bounce_ms)) / 2 / 1e3
+    achieved = (bytes_p + bytes_b) / 2 / avg_launch_s / 1e9
+    roofline = {
+        "bound": "hbm",
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": None,
+    }
+    traffic_file = ROOT / "profiles" / "traffic_cast_c2.json"
+    if traffic_file.exists():
+        try:
+            tr = json.loads(traffic_file.read_text())
+            if tr.get("tile") == args.tile and tr.get("schedule") == args.schedule:
+                roofline["traffic"] = tr.get("hbm_bytes_per_launch")
+        except (ValueError, OSError):
+            pass
+    log(f"primary {n} rays: {statistics.mean(prim_ms):.3f} ms ({n / statistics.mean(prim_ms) / 1e3:.0f} Mrays/s), "
+        f"iters/ray {cp_h[1] / n:.2f}, descents/ray {cp_h[2] / n:.2f}, hit {cp_h[3] / n:.3f}")
+    log(f"bounce  {nb} rays: {statistics.mean(bounce_ms):.3f} ms ({nb / statistics.mean(bounce_ms) / 1e3:.0f} Mrays/s), "
+        f"iters/ray {cb_h[1] / max(nb, 1):.2f}, descents/ray {cb_h[2] / max(nb, 1):.2f}, hit {cb_h[3] / max(nb, 1):.3f}")
+    log(f"bounce generation {statistics.mean(gen_ms):.3f} ms; step {statistics.mean(step_ms):.3f} ms (events), "
+        f"{ms_per_step:.3f} ms (wall); cast roofline {achieved:.0f} GB/s = {achieved / HBM_PEAK_GBS:.1%} of 8 TB/s")
+
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import oracle as O  # the CPU baseline (oracle's SIMD port of the reference CPU cast)
+
+        log("CPU baseline: building the reference's dense world layout")
+        ow = O.OracleWorld.from_stream(scene.stream())
+        po = prim.o.cpu().numpy()
+        pd = prim.d.cpu().numpy()
+        bo = bounce_rays.o[:, :nb].cpu().numpy()
+        bd = bounce_rays.d[:, :nb].cpu().numpy()
+        cores = O.num_cores()
+        ow.cast_simd(po, pd, prim.world_origin, W.MAX_ITERS, threads=cores)  # warm-up
+        times = []
+        for _ in range(args.cpu_runs):
+            t0 = time.perf_counter()
+            rp = ow.cast_simd(po, pd, prim.world_origin, W.MAX_ITERS, threads=cores)
+            rb = ow.cast_simd(bo, bd, prim.world_origin, W.MAX_ITERS, threads=cores)
+            times.append(time.perf_counter() - t0)
+        med = statistics.median(times)
+        # same rays, same answers: spot-check the GPU against the CPU path
+        g_rec = hits_p.rec.cpu().numpy()
+        ok = np.array_equal(g_rec, rp["rec"]) and np.array_equal(hits_b.rec[:nb].cpu().numpy(), rb["rec"])
+        cpu = {
+            "value": round((n + nb) / med / 1e6, 2),
+            "unit": "Mrays/s",
+            "cores": cores,
+            "kind": "port",
+            "sample": f"the full step: {n} primary + {nb} bounce rays, SIMD width {rp['lanes']}, "
+                      f"median of {args.cpu_runs} runs, traversal only",
+            "parity_with_gpu": bool(ok),
+        }
+        log(f"CPU {cpu['value']} Mrays/s on {cores} cores; GPU/CPU parity {ok}")
+
+    line = {
+        "metric": "Mrays/s (1024^3, primary+diffuse)",
+        "value": round(value, 1),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "1024^3 fBm terrain + 3D value-noise caves, 1920x1080 primary + 1 diffuse bounce per hit",
+            "world_bricks": int(info.num_bricks),
+            "primary_rays": n,
+            "bounce_rays": nb,
+            "max_iters": W.MAX_ITERS,
+            "tile": args.tile,
+            "schedule": args.schedule,
+            "parallelism": "single GPU",
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -167,6 +167,8 @@ typedef struct vrt_cast_opts {
     uint32_t chunk;             /* rays per queue grab, multiple of 64 (persistent only, 0 = default) */
     uint32_t waves_per_cu;      /* persistent occupancy target (0 = default) */
     vrt_cast_counters* counters;/* optional device pointer, accumulated (not cleared) */
+    const uint32_t* count;      /* optional device pointer: rays to trace, read when the kernel
+                                   runs (rays->n is then the capacity and the upper bound) */
 } vrt_cast_opts;
 
 int vrt_cast_ex(const vrt_world* world, const vrt_rays* rays, uint32_t max_iters,

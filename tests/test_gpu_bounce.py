@@ -1,0 +1,107 @@
+"""GPU parity of the primary + diffuse pipeline (vrt_cast -> vrt_bounce_generate -> vrt_cast)."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import pytest
+import torch
+
+from helpers import assert_hits_equal
+
+pytestmark = pytest.mark.gpu
+
+
+def run_pipeline(V, world, prim, width, seed, max_iters, schedule=1):
+    hits = world.cast(prim, max_iters, with_pos=True, schedule=schedule)
+    br, src, count = V.bounce(prim, hits, width, seed)
+    hb = world.cast(br, max_iters, schedule=schedule, count=count)
+    torch.cuda.synchronize()
+    nb = int(count.item())
+    return hits, br, src, nb, hb
+
+
+@pytest.mark.parametrize("config", ["c1", "c2"])
+def test_primary_and_bounce_parity(vrt, oracle_mod, cuda, config):
+    import voxelrt_amd as V
+    from voxelrt_amd import workloads as W
+
+    if config == "c1":
+        scene, cam = W.C1_WORLD.build(), W.c1_camera()
+    else:
+        scene, cam = W.C2_WORLD.build(), W.c2_camera()
+    world = V.World.from_scene(scene)
+    ow = oracle_mod.OracleWorld.from_stream(scene.stream())
+    prim = cam.rays_device()
+    hits, br, src, nb, hb = run_pipeline(V, world, prim, cam.c.width, W.BOUNCE_SEED, W.MAX_ITERS)
+    po, pd = prim.o.cpu().numpy(), prim.d.cpu().numpy()
+    ref = ow.cast_simd(po, pd, prim.world_origin, W.MAX_ITERS)
+    ref_s = ow.cast(po, pd, prim.world_origin, W.MAX_ITERS, with_pos=True, begin=0, end=min(po.shape[1], 200000))
+    g_t, g_rec, g_pos = hits.t.cpu().numpy(), hits.rec.cpu().numpy(), hits.pos.cpu().numpy()
+    assert_hits_equal(g_t, g_rec, ref["t"], ref["rec"], f"{config} primary")
+    m = ref_s["rec"].shape[0]
+    hit = (g_rec[:m, 3] & 0xFF) != 0
+    assert np.array_equal(g_pos[:, :m][:, hit].view(np.uint32), ref_s["pos"][:, hit].view(np.uint32))
+
+    # bounce rays: oracle restatement, the library's host twin and the device kernel agree bit-exactly
+    oo, od, osrc = oracle_mod.bounce(g_rec, g_pos, pd, cam.c.width, W.BOUNCE_SEED)
+    assert nb == len(osrc)
+    assert np.array_equal(src[:nb].cpu().numpy().astype(np.uint32), osrc)
+    assert np.array_equal(br.o[:, :nb].cpu().numpy().view(np.uint32), oo.view(np.uint32))
+    assert np.array_equal(br.d[:, :nb].cpu().numpy().view(np.uint32), od.view(np.uint32))
+
+    # bounce casts
+    rb = ow.cast_simd(oo, od, prim.world_origin, W.MAX_ITERS)
+    assert_hits_equal(hb.t[:nb].cpu().numpy(), hb.rec[:nb].cpu().numpy(), rb["t"], rb["rec"], f"{config} bounce")
+    # bounce rays start next to a solid face and go away from it: cosine-weighted around N
+    n_code = g_rec[osrc, 3]
+    N = np.stack([(((n_code >> (8 + 2 * a)) & 3) ^ 2) - 2 for a in range(3)]).astype(np.float64)
+    N /= np.linalg.norm(N, axis=0)
+    cosv = np.sum(N * od, axis=0)
+    assert cosv.min() > -1e-6
+    assert abs(cosv.mean() - 2 / 3) < 0.02
+
+
+def test_host_twin_of_bounce(vrt, cuda):
+    import voxelrt_amd as V
+    from voxelrt_amd import _lib as L
+    from voxelrt_amd import workloads as W
+
+    scene, cam = W.C1_WORLD.build(), W.c1_camera()
+    world = V.World.from_scene(scene)
+    prim = cam.rays_device()
+    hits, br, src, nb, hb = run_pipeline(V, world, prim, cam.c.width, 7, W.MAX_ITERS)
+    n = prim.n
+    rec = np.ascontiguousarray(hits.rec.cpu().numpy())
+    pos = np.ascontiguousarray(hits.pos.cpu().numpy())
+    out = np.zeros((6, n), np.float32)
+    hsrc = np.zeros(n, np.uint32)
+    cnt = C.c_uint32()
+    ptrs = (C.c_void_p * 6)(*[out[i].ctypes.data for i in range(6)])
+    L.check(V.lib().vrt_bounce_host(n, rec.ctypes.data, pos.ctypes.data, cam.c.width, 7, C.byref(ptrs),
+                                    hsrc.ctypes.data, C.byref(cnt)))
+    assert cnt.value == nb
+    assert np.array_equal(out[:3, :nb].view(np.uint32), br.o[:, :nb].cpu().numpy().view(np.uint32))
+    assert np.array_equal(out[3:, :nb].view(np.uint32), br.d[:, :nb].cpu().numpy().view(np.uint32))
+
+
+def test_device_count_bounds_the_cast(vrt, oracle_mod, cuda):
+    """A cast given a device count traces exactly that many rays and leaves the rest untouched."""
+    import voxelrt_amd as V
+    from voxelrt_amd import workloads as W
+
+    scene, cam = W.C1_WORLD.build(), W.c1_camera()
+    world = V.World.from_scene(scene)
+    prim = cam.rays_device()
+    for sched in (1, 2):
+        for k in (0, 1, 777, prim.n):
+            cnt = torch.tensor([k], dtype=torch.int32, device="cuda")
+            h = V.HitBatch.empty(prim.n)
+            h.rec.fill_(-7)
+            ctr = V.new_counters()
+            world.cast(prim, W.MAX_ITERS, schedule=sched, count=cnt, out=h, counters=ctr)
+            torch.cuda.synchronize()
+            rec = h.rec.cpu().numpy()
+            assert int(ctr[0].item()) == k
+            assert np.all(rec[k:] == -7)
+            assert not np.any(np.all(rec[:k] == -7, axis=1))

@@ -82,7 +82,7 @@ class CastCounters(C.Structure):
 
 class CastOpts(C.Structure):
     _fields_ = [("schedule", u32), ("rays_per_lane", u32), ("chunk", u32), ("waves_per_cu", u32),
-                ("counters", P)]
+                ("counters", P), ("count", P)]
 
 
 class HitF64(C.Structure):

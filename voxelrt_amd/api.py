@@ -274,11 +274,14 @@ class World:
 
     def cast(self, rays: RayBatch, max_iters: int = 128, with_pos: bool = False, schedule: int = 0,
              chunk: int = 0, waves_per_cu: int = 0, counters: torch.Tensor | None = None,
-             out: HitBatch | None = None, stream=None) -> HitBatch:
+             out: HitBatch | None = None, count: torch.Tensor | None = None, stream=None) -> HitBatch:
+        """Trace rays (RayCast, CpuRenderer.cpp:172-224).  count: optional device int32 holding the
+        number of rays to trace (rays.n is then the capacity)."""
         hits = out if out is not None else HitBatch.empty(rays.n, with_pos, rays.o.device)
         r = rays.to_c()
         h = hits.to_c()
-        opts = L.CastOpts(schedule, 1, chunk, waves_per_cu, counters.data_ptr() if counters is not None else 0)
+        opts = L.CastOpts(schedule, 1, chunk, waves_per_cu, counters.data_ptr() if counters is not None else 0,
+                          count.data_ptr() if count is not None else 0)
         L.check(L.lib().vrt_cast_ex(self._h, C.byref(r), max_iters, C.byref(h), C.byref(opts),
                                     _stream_handle(stream)))
         return hits

@@ -74,7 +74,7 @@ template <bool COUNT>
 __global__ __launch_bounds__(256) void cast_grid_kernel(WorldView w, RaysIn r, HitsOut h, uint32_t max_iters,
                                                         vrt_cast_counters* ctr) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    bool valid = i < r.n;
+    bool valid = i < ray_count(r);
     RayState st;
     float t = 0.0f;
     int4 rec = make_int4(0, 0, 0, 0);
@@ -95,7 +95,7 @@ template <bool COUNT>
 __global__ __launch_bounds__(256) void cast_persistent_kernel(WorldView w, RaysIn r, HitsOut h, uint32_t max_iters,
                                                               uint32_t chunk, uint32_t refill_min, uint32_t* queue,
                                                               vrt_cast_counters* ctr) {
-    const uint32_t n = r.n;
+    const uint32_t n = ray_count(r);
     uint32_t pool_next = 0, pool_end = 0;  // wave-uniform
     bool exhausted = false;                // wave-uniform: the global queue is drained
     bool has = false;
@@ -244,6 +244,7 @@ int check_rays_hits(const vrt_rays* rays, const vrt_hits* hits, RaysIn& r, HitsO
     r.dx = rays->dx; r.dy = rays->dy; r.dz = rays->dz;
     r.W[0] = rays->world_origin[0]; r.W[1] = rays->world_origin[1]; r.W[2] = rays->world_origin[2];
     r.n = rays->n;
+    r.count = nullptr;
     h.t = hits->t;
     h.rec = (int4*)hits->rec;
     if (hits->pos) {
@@ -359,6 +360,7 @@ int vrt_cast_ex(const vrt_world* world, const vrt_rays* rays, uint32_t max_iters
         vrt_cast_opts o;
         std::memset(&o, 0, sizeof(o));
         if (opts) o = *opts;
+        r.count = o.count;
         return launch_cast(world, r, h, max_iters, o, (hipStream_t)stream);
     });
 }

@@ -16,8 +16,14 @@ struct RaysIn {
     const float* __restrict__ dy;
     const float* __restrict__ dz;
     int32_t W[3];
-    uint32_t n;
+    uint32_t n;                        // capacity
+    const uint32_t* __restrict__ count;  // optional device ray count (<= n)
 };
+
+// Rays to trace: the device count when given, clamped to the capacity.
+__device__ __forceinline__ uint32_t ray_count(const RaysIn& r) {
+    return r.count ? min(*r.count, r.n) : r.n;
+}
 
 struct HitsOut {
     float* __restrict__ t;
