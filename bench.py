@@ -57,6 +57,7 @@ def main() -> int:
     ap.add_argument("--tile", default="8x8")
     ap.add_argument("--schedule", default="grid", choices=["grid", "persistent"])
     ap.add_argument("--chunk", type=int, default=0)
+    ap.add_argument("--refill", type=int, default=0)
     ap.add_argument("--cpu-runs", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-mb", type=int, default=0,
@@ -108,7 +109,7 @@ def main() -> int:
     sh = stream.cuda_stream
 
     def opts(counters=None, cnt=None):
-        return L.CastOpts(sched, 1, args.chunk, 0, counters.data_ptr() if counters is not None else 0,
+        return L.CastOpts(sched, args.refill, args.chunk, 0, counters.data_ptr() if counters is not None else 0,
                           cnt.data_ptr() if cnt is not None else 0)
 
     o_plain = opts()

@@ -4,8 +4,8 @@
  * The world is a two-level brickmap: a sector is 4x4x4 bricks (32^3 voxels),
  * a brick is 8x8x8 voxels of one byte each (0 = empty, 1..255 = material id).
  * On the device it is held as structure-of-arrays buffers in HBM:
- *   sector masks  u64 [nsec]       bit b set = brick b of the sector is allocated
- *   base slots    u32 [nsec]       slot of the sector's first allocated brick
+ *   sectors       16 B [nsec]      allocation mask (bit b = brick b allocated) and
+ *                                  the slot of the sector's first allocated brick
  *   occupancy     u64 [slots * 8]  per brick, one mask per 4x4x4 voxel cell
  *   voxels        u8  [slots * 512]
  *   palette       u64 [256]
@@ -99,10 +99,16 @@ int vrt_world_sync(vrt_world* world, uint32_t nsec, const uint32_t* sector_idx,
 /* Palette entries (Material::GetEncoded layout, VoxelMap.h:27-41). Host pointer, 256 entries. */
 int vrt_world_set_palette(vrt_world* world, const uint64_t* palette, void* stream);
 
+/* One sector (GpuRenderer.cpp:27-33 AllocMasks + BaseSlots, packed so a lookup is one 16-byte load). */
+typedef struct vrt_sector_record {
+    uint64_t mask;       /* bit b: brick b (x | z << 2 | y << 4 in the sector) is allocated */
+    uint32_t base_slot;  /* slot of the first allocated brick */
+    uint32_t reserved;
+} vrt_sector_record;
+
 /* Device views of the world buffers (read-only use by callers). */
 typedef struct vrt_world_buffers {
-    const uint64_t* sector_mask;  /* [num_sectors] */
-    const uint32_t* base_slot;    /* [num_sectors] */
+    const vrt_sector_record* sectors; /* [num_sectors] */
     const uint64_t* occupancy;    /* [slot_capacity * 8] */
     const uint8_t* voxels;        /* [slot_capacity * 512] */
     const uint64_t* palette;      /* [256] */
@@ -163,7 +169,7 @@ typedef enum vrt_cast_schedule {
 
 typedef struct vrt_cast_opts {
     uint32_t schedule;          /* vrt_cast_schedule */
-    uint32_t rays_per_lane;     /* 1 or 2 independent rays interleaved per lane (persistent only) */
+    uint32_t refill;            /* persistent: free lanes that trigger a retire/refill, 1..64 (0 = 16) */
     uint32_t chunk;             /* rays per queue grab, multiple of 64 (persistent only, 0 = default) */
     uint32_t waves_per_cu;      /* persistent occupancy target (0 = default) */
     vrt_cast_counters* counters;/* optional device pointer, accumulated (not cleared) */
@@ -261,17 +267,20 @@ typedef struct vrt_segment {
 } vrt_segment;
 
 typedef struct vrt_batch {
-    const vrt_camera* cameras; uint32_t ncameras;  /* host */
+    const vrt_camera* cameras; uint32_t ncameras;  /* host; each camera's tile fixes its hit order */
     const vrt_segment* segments; uint32_t nsegments; /* host */
     vrt_rays rays;           /* buffers for VRT_SEG_RAYS segments (world_origin, n unused) */
-    vrt_hits hits;           /* output, indexed by hit_offset */
+    vrt_hits hits;           /* output, indexed by hit_offset; pos (optional) is [3][H] with
+                                H = max over segments of hit_offset + count */
     uint32_t max_iters;
 } vrt_batch;
 
 typedef struct vrt_batch_tuning {
-    uint32_t tile_w, tile_h;   /* lane -> pixel tile for camera segments: 8x8 or 16x4 */
-    uint32_t rays_per_lane;    /* 1 or 2 */
-    uint32_t chunk;            /* rays per queue grab, multiple of 64 */
+    uint32_t tile_w, tile_h;   /* lane -> pixel tile inside camera segments: 8x8, 16x4, 4x16 or 64x1
+                                  (camera sizes must be multiples of it); the hit order does not change */
+    uint32_t rays_per_lane;    /* 1 or 2 rays traced interleaved per lane */
+    uint32_t chunk;            /* 0: one wave per 64*rays_per_lane rays, balanced by the hardware
+                                  dispatcher; else persistent waves claim this many rays per grab */
 } vrt_batch_tuning;
 
 typedef struct vrt_batch_plan vrt_batch_plan;
@@ -287,7 +296,7 @@ int vrt_batch_plan_destroy(vrt_batch_plan* plan);
 
 /*
  * Time every tuning in the search space on this batch (tile 8x8/16x4, rays per
- * lane 1/2, chunk 64..1024) and return the fastest.  Results are cached per
+ * lane 1/2, chunk 0/64/256) and return the fastest.  Results are cached per
  * device name and batch signature in-process and, when cache_path is not NULL,
  * in that text file.
  */

@@ -219,22 +219,26 @@ uint32_t oracle_bounce(uint32_t n, const float* t, const oracle_hit* rec, const 
 }
 
 /* ---------------------------------------------------------------------------
- * Lambertian render gradient: radiance_pc = albedo[v_p][c] * w_p,
- * L = sum (radiance - target)^2, dL/dalbedo[v][c] = sum_p 2 (radiance_pc - target_pc) w_p.
+ * Backward of the Lambertian render: radiance_pc = albedo[v_p][c] * w_p, so
+ * dL/dalbedo[v][c] = sum over pixels p with v_p = v of dL/dradiance_pc * w_p.
+ * FP64 sums; abssum[v][c] = sum of |terms|, the scale of the rounding error of
+ * any fp32 evaluation of the same sum.
  */
-void oracle_render_grad(uint32_t npix, const uint32_t* saved_voxel, const float* saved_weight,
-                        const float* albedo, const float* target, double* grad) {
+void oracle_render_backward(uint32_t npix, const uint32_t* saved_voxel, const float* saved_weight,
+                            const float* grad_image, double* grad, double* abssum) {
     for (uint32_t p = 0; p < npix; p++) {
         uint32_t v = saved_voxel[p];
-        double wgt = (double)saved_weight[p];
         if (v == 0xFFFFFFFFu) continue;
+        double wgt = (double)saved_weight[p];
         for (int c = 0; c < 3; c++) {
-            double rad = (double)albedo[(size_t)v * 3 + c] * wgt;
-            grad[(size_t)v * 3 + c] += 2.0 * (rad - (double)target[(size_t)p * 3 + c]) * wgt;
+            double term = (double)grad_image[(size_t)p * 3 + c] * wgt;
+            grad[(size_t)v * 3 + c] += term;
+            abssum[(size_t)v * 3 + c] += fabs(term);
         }
     }
 }
 
+/* L = sum_p sum_c (albedo[v_p][c] * w_p - target_pc)^2 in FP64 (0 radiance on a miss). */
 double oracle_render_loss(uint32_t npix, const uint32_t* saved_voxel, const float* saved_weight,
                           const double* albedo, const float* target) {
     double loss = 0.0;

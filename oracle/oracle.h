@@ -111,10 +111,11 @@ uint32_t oracle_bounce(uint32_t n, const float* t, const oracle_hit* rec, const 
                        const float* const d[3], uint32_t width, uint32_t seed,
                        float* const out_o[3], float* const out_d[3], uint32_t* src);
 
-/* Reference gradient of L = sum_p sum_c (radiance - target)^2 for the Lambertian
- * render, FP64: grad[v*3+c] += 2 (radiance_pc - target_pc) * weight_p. */
-void oracle_render_grad(uint32_t npix, const uint32_t* saved_voxel, const float* saved_weight,
-                        const float* albedo, const float* target, double* grad);
+/* Backward of the Lambertian render in FP64: grad[v*3+c] += grad_image[p*3+c] * weight_p
+ * over pixels p that hit voxel v; abssum accumulates |terms| (the error scale of an fp32 sum). */
+void oracle_render_backward(uint32_t npix, const uint32_t* saved_voxel, const float* saved_weight,
+                            const float* grad_image, double* grad, double* abssum);
+/* L = sum_p sum_c (albedo[v_p][c] * weight_p - target_pc)^2 in FP64. */
 double oracle_render_loss(uint32_t npix, const uint32_t* saved_voxel, const float* saved_weight,
                           const double* albedo, const float* target);
 

@@ -63,7 +63,7 @@ def lib() -> C.CDLL:
         h.oracle_bounce.argtypes = [u32, P, P, C.POINTER(P * 3), C.POINTER(P * 3), u32, u32, C.POINTER(P * 3),
                                     C.POINTER(P * 3), P]
         h.oracle_bounce.restype = u32
-        h.oracle_render_grad.argtypes = [u32, P, P, P, P, P]
+        h.oracle_render_backward.argtypes = [u32, P, P, P, P, P]
         h.oracle_render_loss.argtypes = [u32, P, P, P, P]
         h.oracle_render_loss.restype = C.c_double
         _lib = h
@@ -205,14 +205,16 @@ def bounce(rec: np.ndarray, pos: np.ndarray, d: np.ndarray, width: int, seed: in
     return oo[:, :k].copy(), od[:, :k].copy(), src[:k].copy()
 
 
-def render_grad(saved_voxel, saved_weight, albedo, target, nvox) -> np.ndarray:
+def render_backward(saved_voxel, saved_weight, grad_image, nvox):
+    """FP64 backward: (grad [nvox, 3], abssum [nvox, 3])."""
     sv = np.ascontiguousarray(saved_voxel, dtype=np.uint32)
     sw = np.ascontiguousarray(saved_weight, dtype=np.float32)
-    al = np.ascontiguousarray(albedo, dtype=np.float32)
-    tg = np.ascontiguousarray(target, dtype=np.float32)
+    gi = np.ascontiguousarray(grad_image, dtype=np.float32)
     g = np.zeros(nvox * 3, dtype=np.float64)
-    lib().oracle_render_grad(len(sv), sv.ctypes.data, sw.ctypes.data, al.ctypes.data, tg.ctypes.data, g.ctypes.data)
-    return g
+    a = np.zeros(nvox * 3, dtype=np.float64)
+    lib().oracle_render_backward(len(sv), sv.ctypes.data, sw.ctypes.data, gi.ctypes.data, g.ctypes.data,
+                                 a.ctypes.data)
+    return g.reshape(-1, 3), a.reshape(-1, 3)
 
 
 def render_loss(saved_voxel, saved_weight, albedo64, target) -> float:

@@ -15,9 +15,17 @@ from .api import (  # noqa: F401
     incoherent_rays_host,
     new_counters,
     bounce,
+    Batch,
+    BatchPlan,
+    autotune,
+    render,
+    render_saved,
+    render_backward,
+    sun_direction,
 )
 
 __all__ = [
     "VrtError", "CapacityError", "lib", "LIB_PATH", "Scene", "World", "RayBatch", "HitBatch", "HitResult",
-    "CameraSpec", "incoherent_rays_host", "new_counters", "bounce",
+    "CameraSpec", "incoherent_rays_host", "new_counters", "bounce", "Batch", "BatchPlan", "autotune", "render",
+    "render_saved", "render_backward", "sun_direction",
 ]

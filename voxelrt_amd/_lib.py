@@ -60,7 +60,7 @@ class WorldInfo(C.Structure):
 
 
 class WorldBuffers(C.Structure):
-    _fields_ = [("sector_mask", P), ("base_slot", P), ("occupancy", P), ("voxels", P), ("palette", P)]
+    _fields_ = [("sectors", P), ("occupancy", P), ("voxels", P), ("palette", P)]
 
 
 class Rays(C.Structure):
@@ -81,7 +81,7 @@ class CastCounters(C.Structure):
 
 
 class CastOpts(C.Structure):
-    _fields_ = [("schedule", u32), ("rays_per_lane", u32), ("chunk", u32), ("waves_per_cu", u32),
+    _fields_ = [("schedule", u32), ("refill", u32), ("chunk", u32), ("waves_per_cu", u32),
                 ("counters", P), ("count", P)]
 
 

@@ -22,7 +22,8 @@ from . import _lib as L
 
 def _stream_handle(stream) -> int | None:
     if stream is None:
-        return torch.cuda.current_stream().cuda_stream
+        # torch's current stream; the null stream when torch sees no device (the library reports the error)
+        return torch.cuda.current_stream().cuda_stream if torch.cuda.is_available() else 0
     if isinstance(stream, torch.cuda.Stream):
         return stream.cuda_stream
     return int(stream)
@@ -73,6 +74,7 @@ class Scene:
         return Scene(out.value)
 
     def stream(self) -> dict:
+        """Sector stream views into the scene's memory; the dict keeps the scene alive."""
         s = L.SectorStream()
         L.check(L.lib().vrt_scene_get(self._h, C.byref(s)))
 
@@ -91,6 +93,7 @@ class Scene:
             "bricks": view(s.bricks, int(s.nbricks) * 512, np.uint8),
             "nbricks": int(s.nbricks),
             "solid_voxels": int(s.solid_voxels),
+            "_owner": self,
         }
 
     def voxel(self, x: int, y: int, z: int) -> int:
@@ -274,13 +277,14 @@ class World:
 
     def cast(self, rays: RayBatch, max_iters: int = 128, with_pos: bool = False, schedule: int = 0,
              chunk: int = 0, waves_per_cu: int = 0, counters: torch.Tensor | None = None,
-             out: HitBatch | None = None, count: torch.Tensor | None = None, stream=None) -> HitBatch:
+             out: HitBatch | None = None, count: torch.Tensor | None = None, refill: int = 0,
+             stream=None) -> HitBatch:
         """Trace rays (RayCast, CpuRenderer.cpp:172-224).  count: optional device int32 holding the
         number of rays to trace (rays.n is then the capacity)."""
         hits = out if out is not None else HitBatch.empty(rays.n, with_pos, rays.o.device)
         r = rays.to_c()
         h = hits.to_c()
-        opts = L.CastOpts(schedule, 1, chunk, waves_per_cu, counters.data_ptr() if counters is not None else 0,
+        opts = L.CastOpts(schedule, refill, chunk, waves_per_cu, counters.data_ptr() if counters is not None else 0,
                           count.data_ptr() if count is not None else 0)
         L.check(L.lib().vrt_cast_ex(self._h, C.byref(r), max_iters, C.byref(h), C.byref(opts),
                                     _stream_handle(stream)))
@@ -326,3 +330,155 @@ def bounce(rays: RayBatch, hits: HitBatch, width: int, seed: int, stream=None):
     L.check(L.lib().vrt_bounce_generate(C.byref(r), C.byref(h), width, seed, C.byref(bufs), src.data_ptr(),
                                         count.data_ptr(), scratch.data_ptr(), _stream_handle(stream)))
     return out, src, count
+
+
+# ------------------------------------------------------------------ batched
+
+
+class Batch:
+    """Camera segments (rays built in the kernel) followed by ray segments (read from `rays`),
+    one hit range per segment in that order (vrt_batch).
+
+    ray_segments: (ray_offset, count, world_origin, count_tensor_or_None); with a device count,
+    `count` is the capacity.
+    """
+
+    def __init__(self, cameras: list, ray_segments: list = (), rays: RayBatch | None = None, max_iters: int = 128,
+                 with_pos: bool = False, device="cuda"):
+        self.cameras = list(cameras)
+        self.ray_segments = list(ray_segments)
+        self.rays = rays
+        self.max_iters = max_iters
+        segs = []
+        off = 0
+        self.ranges = []
+        for k, cam in enumerate(self.cameras):
+            segs.append(L.Segment(L.VRT_SEG_CAMERA, k, 0, off, cam.n, (C.c_int32 * 3)(0, 0, 0), 0))
+            self.ranges.append((off, cam.n))
+            off += cam.n
+        for (ro, cnt, W, cptr) in self.ray_segments:
+            segs.append(L.Segment(L.VRT_SEG_RAYS, 0, ro, off, cnt, (C.c_int32 * 3)(*W),
+                                  cptr.data_ptr() if cptr is not None else 0))
+            self.ranges.append((off, cnt))
+            off += cnt
+        self.total = off
+        self.hits = HitBatch.empty(off, with_pos, device)
+        self._cams = (L.Camera * max(1, len(self.cameras)))(*[c.c for c in self.cameras])
+        self._segs = (L.Segment * max(1, len(segs)))(*segs)
+        self._nseg = len(segs)
+
+    def to_c(self) -> L.Batch:
+        r = self.rays.to_c() if self.rays is not None else L.Rays()
+        return L.Batch(C.cast(self._cams, C.c_void_p), len(self.cameras), C.cast(self._segs, C.c_void_p),
+                       self._nseg, r, self.hits.to_c(), self.max_iters)
+
+    def segment_hits(self, k: int):
+        off, cnt = self.ranges[k]
+        return (self.hits.t[off:off + cnt], self.hits.rec[off:off + cnt],
+                self.hits.pos[:, off:off + cnt] if self.hits.pos is not None else None)
+
+
+class BatchPlan:
+    """vrt_batch_plan: device tables for one batch and tuning; launch or graph-launch it."""
+
+    def __init__(self, world: World, batch: Batch, tuning=(8, 8, 1, 0), stream=None):
+        self.batch = batch
+        self._b = batch.to_c()
+        t = L.BatchTuning(*tuning)
+        out = C.c_void_p()
+        L.check(L.lib().vrt_batch_plan_create(world.handle, C.byref(self._b), C.byref(t), _stream_handle(stream),
+                                              C.byref(out)))
+        self._h = out
+
+    def launch(self, counters: torch.Tensor | None = None, stream=None) -> None:
+        L.check(L.lib().vrt_batch_launch(self._h, counters.data_ptr() if counters is not None else 0,
+                                         _stream_handle(stream)))
+
+    def graph_launch(self, stream=None) -> None:
+        L.check(L.lib().vrt_batch_graph_launch(self._h, _stream_handle(stream)))
+
+    def close(self) -> None:
+        if self._h and self._h.value:
+            L.check(L.lib().vrt_batch_plan_destroy(self._h))
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def autotune(world: World, batch: Batch, cache_path: str | None = None, stream=None):
+    """vrt_batch_autotune: returns ((tile_w, tile_h, rays_per_lane, chunk), ms)."""
+    b = batch.to_c()
+    best = L.BatchTuning()
+    ms = C.c_float()
+    L.check(L.lib().vrt_batch_autotune(world.handle, C.byref(b), cache_path.encode() if cache_path else None,
+                                       C.byref(best), C.byref(ms), _stream_handle(stream)))
+    return (best.tile_w, best.tile_h, best.rays_per_lane, best.chunk), ms.value
+
+
+# ------------------------------------------------------------------ differentiable
+
+SUN_DIR = (0.3, 0.9, -0.28)   # VoxelRender.comp:133, normalised below
+SUN_INTENSITY = 5.0           # VoxelRender.comp:134
+
+
+def sun_direction() -> tuple:
+    x, y, z = SUN_DIR
+    n = (x * x + y * y + z * z) ** 0.5
+    return (x / n, y / n, z / n)
+
+
+_torch_ext = None
+
+
+def torch_ext():
+    """The libtorch autograd extension (_vrt_torch.so, built next to libvoxelrt_hip.so)."""
+    global _torch_ext
+    if _torch_ext is None:
+        import importlib.util
+        from pathlib import Path
+
+        path = Path(__file__).resolve().parent / "_vrt_torch.so"
+        if not path.exists():
+            raise RuntimeError(f"{path} is missing: build it with `python -m voxelrt_amd.build`")
+        L.lib()  # load libvoxelrt_hip.so first
+        spec = importlib.util.spec_from_file_location("_vrt_torch", str(path))
+        mod = importlib.util.module_from_spec(spec)
+        spec.loader.exec_module(mod)
+        _torch_ext = mod
+    return _torch_ext
+
+
+def _pack_cameras(cameras) -> torch.Tensor:
+    raw = b"".join(bytes(c.c) for c in cameras)
+    return torch.frombuffer(bytearray(raw), dtype=torch.uint8)
+
+
+def render(world: World, albedo: torch.Tensor, cameras, max_iters: int = 128, light=None,
+           intensity: float = SUN_INTENSITY, stream=None) -> torch.Tensor:
+    """Differentiable Lambertian render: [pixels, 3] image, gradients flow to `albedo`
+    ([voxel slots * 512, 3] fp32) through the libtorch autograd Function."""
+    light = light if light is not None else sun_direction()
+    return torch_ext().render(albedo, world.handle.value, _pack_cameras(cameras), max_iters, list(light),
+                              float(intensity), _stream_handle(stream))
+
+
+def render_saved(world: World, albedo: torch.Tensor, cameras, max_iters: int = 128, light=None,
+                 intensity: float = SUN_INTENSITY, stream=None):
+    """Forward render returning (image, saved voxel ids int32 (-1 = miss), saved weights)."""
+    light = light if light is not None else sun_direction()
+    return torch_ext().render_saved(albedo, world.handle.value, _pack_cameras(cameras), max_iters, list(light),
+                                    float(intensity), _stream_handle(stream))
+
+
+def render_backward(world: World, grad_image: torch.Tensor, saved_voxel: torch.Tensor, saved_weight: torch.Tensor,
+                    nvox: int, stream=None) -> torch.Tensor:
+    """vrt_render_bwd into a fresh zero gradient [nvox, 3]."""
+    grad = torch.zeros((nvox, 3), dtype=torch.float32, device=grad_image.device)
+    g = grad_image.contiguous()
+    L.check(L.lib().vrt_render_bwd(world.handle, saved_voxel.numel(), g.data_ptr(), saved_voxel.data_ptr(),
+                                   saved_weight.data_ptr(), grad.data_ptr(), _stream_handle(stream)))
+    return grad

@@ -68,6 +68,35 @@ def build_library(verbose: bool = True) -> Path:
     return LIB
 
 
+TORCH_EXT = PKG / "_vrt_torch.so"
+
+
+def build_torch_extension(verbose: bool = True) -> Path:
+    """libtorch autograd wrapper (csrc/torch_render.cpp), a Python extension linked
+    against libvoxelrt_hip.so (found next to it through $ORIGIN)."""
+    import sysconfig
+
+    import torch
+    from torch.utils import cpp_extension
+
+    cxx = shutil.which("g++") or "g++"
+    abi = int(torch._C._GLIBCXX_USE_CXX11_ABI)
+    cmd = [cxx, "-O2", "-fPIC", "-shared", "-std=c++17", f"-D_GLIBCXX_USE_CXX11_ABI={abi}",
+           "-DTORCH_EXTENSION_NAME=_vrt_torch", "-DTORCH_API_INCLUDE_EXTENSION_H",
+           f"-I{ROOT / 'include'}", f"-I{sysconfig.get_paths()['include']}"]
+    cmd += [f"-I{p}" for p in cpp_extension.include_paths()]
+    cmd += [str(CSRC / "torch_render.cpp"), "-o", str(TORCH_EXT) + ".tmp"]
+    cmd += [f"-L{p}" for p in cpp_extension.library_paths()]
+    cmd += ["-lc10", "-ltorch", "-ltorch_cpu", "-ltorch_python", f"-L{PKG}", "-lvoxelrt_hip", "-Wl,-rpath,$ORIGIN"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"torch extension build failed:\n{' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
+    os.replace(str(TORCH_EXT) + ".tmp", TORCH_EXT)
+    if verbose:
+        print(f"built {TORCH_EXT.relative_to(ROOT)}", flush=True)
+    return TORCH_EXT
+
+
 def build_oracle(verbose: bool = True) -> Path:
     """The oracle is test infrastructure (the checker and CPU baseline)."""
     make = shutil.which("make")
@@ -83,6 +112,7 @@ def build_oracle(verbose: bool = True) -> Path:
 
 def build_all(verbose: bool = True) -> None:
     build_library(verbose)
+    build_torch_extension(verbose)
     build_oracle(verbose)
 
 

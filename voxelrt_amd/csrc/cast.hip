@@ -54,12 +54,13 @@ __device__ __forceinline__ void store_result(const HitsOut& h, uint32_t i, const
 }
 
 // Traces one ray to completion.
+template <bool EXACT>
 __device__ __forceinline__ void trace_full(const WorldView& w, RayState& st, uint32_t max_iters, float& t, int4& rec) {
     int32_t v[3];
     uint32_t slot = 0;
     int res = STEP_MISS;
     for (uint32_t it = 0; it < max_iters; it++) {
-        res = ray_step(w, st, v, &slot);
+        res = ray_step<EXACT>(w, st, v, &slot);
         if (res != STEP_CONTINUE) break;
     }
     if (res == STEP_HIT)
@@ -80,7 +81,11 @@ __global__ __launch_bounds__(256) void cast_grid_kernel(WorldView w, RaysIn r, H
     int4 rec = make_int4(0, 0, 0, 0);
     if (valid) {
         ray_setup(st, r.ox[i], r.oy[i], r.oz[i], r.dx[i], r.dy[i], r.dz[i], r.W[0], r.W[1], r.W[2]);
-        trace_full(w, st, max_iters, t, rec);
+        // the fast loop when every ray of the wave is tame, else the exact one
+        if (__all(ray_is_tame(st)))
+            trace_full<false>(w, st, max_iters, t, rec);
+        else
+            trace_full<true>(w, st, max_iters, t, rec);
         store_result(h, i, st, t, rec);
     }
     if constexpr (COUNT) {
@@ -91,58 +96,74 @@ __global__ __launch_bounds__(256) void cast_grid_kernel(WorldView w, RaysIn r, H
 
 // ---------------------------------------------------------------- PERSISTENT schedule
 
+// Work distribution: rays are cut into chunks; the chunk range is split into
+// kQueues contiguous parts, one per XCD (workgroup b runs on XCD b % 8), so an
+// XCD's L2 sees a compact part of the image.  Each wave owns its first chunk
+// statically; further chunks come from its XCD's queue counter and, once that
+// part is drained, from the other parts in turn.  The counters live on separate
+// 256-byte lines, so claims from different XCDs do not serialize.
+static const uint32_t kQueues = 8;
+static const uint32_t kQueueStride = 64;  // u32 per counter (256 bytes)
+
+struct ChunkQueue {
+    uint32_t* ctr;          // kQueues counters, kQueueStride apart, zero at launch
+    uint32_t nchunks;       // total chunks
+    uint32_t waves_per_q;   // waves with a static first chunk in each part
+};
+
+__device__ __forceinline__ uint32_t part_begin(uint32_t q, uint32_t nchunks) {
+    return (uint32_t)(((uint64_t)nchunks * q) / kQueues);
+}
+
+// Next chunk for this wave, or ~0u when every part is drained.  Wave-uniform.
+__device__ __forceinline__ uint32_t claim_chunk(const ChunkQueue& cq, uint32_t home, uint32_t& probe) {
+    while (probe < kQueues) {
+        uint32_t q = (home + probe) % kQueues;
+        uint32_t b = part_begin(q, cq.nchunks), e = part_begin(q + 1, cq.nchunks);
+        uint32_t k = 0;
+        if (__lane_id() == 0) k = atomicAdd(cq.ctr + q * kQueueStride, 1u);
+        k = __shfl(k, 0, 64);
+        uint32_t c = b + cq.waves_per_q + k;
+        if (c < e) return c;
+        probe++;
+    }
+    return ~0u;
+}
+
+// Each wave keeps 64 ray slots.  A finished ray parks its result in registers;
+// once at least refill_min lanes are free (or all of them), the wave writes
+// the parked results and loads new rays into the free lanes from its pool of
+// consecutive ray indices.  Retire and refill therefore cost one divergent
+// block per refill_min rays rather than one per ray.
 template <bool COUNT>
 __global__ __launch_bounds__(256) void cast_persistent_kernel(WorldView w, RaysIn r, HitsOut h, uint32_t max_iters,
-                                                              uint32_t chunk, uint32_t refill_min, uint32_t* queue,
+                                                              uint32_t chunk, uint32_t refill_min, ChunkQueue cq,
                                                               vrt_cast_counters* ctr) {
     const uint32_t n = ray_count(r);
+    const uint32_t home = blockIdx.x % kQueues;
+    const uint32_t wave_in_part = (blockIdx.x / kQueues) * (blockDim.x / 64) + threadIdx.x / 64;
+    uint32_t probe = 0;
+    // static first chunk
+    uint32_t first = part_begin(home, cq.nchunks) + wave_in_part;
     uint32_t pool_next = 0, pool_end = 0;  // wave-uniform
-    bool exhausted = false;                // wave-uniform: the global queue is drained
-    bool has = false;
-    uint32_t idx = 0;
+    if (wave_in_part < cq.waves_per_q && first < part_begin(home + 1, cq.nchunks)) {
+        pool_next = min(first * chunk, n);  // chunks past a device-side count are empty
+        pool_end = min(pool_next + chunk, n);
+    }
+    bool exhausted = false;                // wave-uniform: every part is drained
+    bool tame = true;                      // wave-uniform: every ray in the wave is tame
+    bool has = false;                      // the lane holds a ray (running or parked)
+    bool done = false;                     // the lane's ray has finished, result parked
+    int res = STEP_CONTINUE;
+    uint32_t idx = 0, slot = 0;
+    int32_t v[3] = {0, 0, 0};
     RayState st;
     unsigned long long c_rays = 0, c_iters = 0, c_desc = 0, c_hits = 0;
     for (;;) {
-        uint64_t idle = __ballot(!has);
-        uint32_t nidle = (uint32_t)__popcll(idle);
-        if (nidle >= refill_min || nidle == 64) {
-            if (pool_next == pool_end && !exhausted) {
-                uint32_t g = 0;
-                if (__lane_id() == 0) g = atomicAdd(queue, chunk);
-                g = __shfl(g, 0, 64);
-                if (g >= n) {
-                    exhausted = true;
-                } else {
-                    pool_next = g;
-                    pool_end = min(g + chunk, n);
-                }
-            }
-            uint32_t avail = pool_end - pool_next;
-            if (!has) {
-                uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32),
-                                                          __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
-                if (rank < avail) {
-                    idx = pool_next + rank;
-                    ray_setup(st, r.ox[idx], r.oy[idx], r.oz[idx], r.dx[idx], r.dy[idx], r.dz[idx], r.W[0], r.W[1],
-                              r.W[2]);
-                    has = true;
-                }
-            }
-            pool_next += min(avail, nidle);
-        }
-        if (!__any(has)) {
-            if (exhausted || pool_next == pool_end) {
-                // nothing in flight: either all work is done or the pool is empty and
-                // the next iteration grabs a new chunk
-                if (exhausted) break;
-                continue;
-            }
-        }
-        if (has) {
-            int32_t v[3];
-            uint32_t slot = 0;
-            int res = st.iters < max_iters ? ray_step(w, st, v, &slot) : STEP_MISS;
-            if (res != STEP_CONTINUE) {
+        uint64_t freem = __ballot(!has || done);
+        uint32_t nfree = (uint32_t)__popcll(freem);
+        if (nfree >= refill_min) {
+            if (has && done) {
                 float t;
                 int4 rec;
                 if (res == STEP_HIT)
@@ -157,6 +178,48 @@ __global__ __launch_bounds__(256) void cast_persistent_kernel(WorldView w, RaysI
                     c_hits += (rec.w & 0xFF) != 0;
                 }
                 has = false;
+            }
+            if (pool_next == pool_end && !exhausted) {
+                uint32_t c = claim_chunk(cq, home, probe);
+                if (c == ~0u) {
+                    exhausted = true;
+                } else {
+                    pool_next = min(c * chunk, n);
+                    pool_end = min(pool_next + chunk, n);
+                }
+            }
+            uint32_t avail = pool_end - pool_next;
+            if (!has) {
+                uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(freem >> 32),
+                                                          __builtin_amdgcn_mbcnt_lo((uint32_t)freem, 0u));
+                if (rank < avail) {
+                    idx = pool_next + rank;
+                    ray_setup(st, r.ox[idx], r.oy[idx], r.oz[idx], r.dx[idx], r.dy[idx], r.dz[idx], r.W[0], r.W[1],
+                              r.W[2]);
+                    has = true;
+                    done = false;
+                }
+            }
+            pool_next += min(avail, nfree);
+            tame = __ballot(has && !ray_is_tame(st)) == 0;
+            if (!__any(has)) {
+                if (exhausted) break;
+                continue;
+            }
+        }
+        if (has && !done) {
+            // two steps per check of the free-lane count
+#pragma unroll
+            for (int k = 0; k < 2; k++) {
+                if (!done) {
+                    if (st.iters >= max_iters)
+                        res = STEP_MISS;
+                    else if (tame)
+                        res = ray_step<false>(w, st, v, &slot);
+                    else
+                        res = ray_step<true>(w, st, v, &slot);
+                    done = res != STEP_CONTINUE;
+                }
             }
         }
     }
@@ -178,7 +241,7 @@ struct DeviceInfo {
 static DeviceInfo g_dev[64];
 static std::mutex g_dev_mu;
 
-static const uint32_t kQueueRing = 4096;
+static const uint32_t kQueueRing = 256;  // concurrent persistent launches per device
 
 int device_info(int dev, int* cus, uint32_t** queue) {
     if (dev < 0 || dev >= 64) return fail(VRT_ERR_NO_DEVICE, "device index out of range");
@@ -189,11 +252,11 @@ int device_info(int dev, int* cus, uint32_t** queue) {
             hipDeviceProp_t prop;
             VRT_HIP(hipGetDeviceProperties(&prop, dev));
             di.cus = prop.multiProcessorCount;
-            VRT_HIP(hipMalloc(&di.queues, kQueueRing * sizeof(uint32_t)));
+            VRT_HIP(hipMalloc(&di.queues, (size_t)kQueueRing * kQueues * kQueueStride * sizeof(uint32_t)));
         }
     }
     *cus = di.cus;
-    if (queue) *queue = di.queues + (di.next.fetch_add(1) % kQueueRing);
+    if (queue) *queue = di.queues + (size_t)(di.next.fetch_add(1) % kQueueRing) * kQueues * kQueueStride;
     return VRT_OK;
 }
 
@@ -214,22 +277,26 @@ int launch_cast(const vrt_world* world, const RaysIn& r, const HitsOut& h, uint3
         return VRT_OK;
     }
     if (sched != VRT_SCHED_PERSISTENT) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_cast_ex: unknown schedule");
-    uint32_t chunk = o.chunk ? o.chunk : 256;
+    uint32_t chunk = o.chunk ? o.chunk : 64;
     if (chunk % 64 != 0 || chunk > (1u << 20)) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_cast_ex: chunk must be a multiple of 64");
+    if (r.n > 0x7FFFFFFFu) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_cast_ex: persistent schedule takes at most 2^31 - 1 rays");
     int cus = 0;
     uint32_t* queue = nullptr;
     int rc = device_info(world->device, &cus, &queue);
     if (rc != VRT_OK) return rc;
-    uint32_t wpc = o.waves_per_cu ? o.waves_per_cu : 32;
-    uint32_t blocks = std::max<uint32_t>(1, (uint32_t)cus * wpc / 4);
-    uint32_t max_blocks = (r.n + 255) / 256;
-    blocks = std::min(blocks, std::max<uint32_t>(1, max_blocks));
-    uint32_t refill_min = o.rays_per_lane >= 64 ? 64 : 16;
-    fill_u32_kernel<<<1, 64, 0, s>>>(queue, 1, 0u);
+    uint32_t wpc = o.waves_per_cu ? std::min<uint32_t>(o.waves_per_cu, 32) : 32;
+    // whole rounds of kQueues workgroups so every XCD part gets the same number of waves
+    uint32_t blocks = std::max<uint32_t>(kQueues, ((uint32_t)cus * wpc / 4) / kQueues * kQueues);
+    ChunkQueue cq;
+    cq.ctr = queue;
+    cq.nchunks = (r.n + chunk - 1) / chunk;
+    cq.waves_per_q = blocks / kQueues * 4;
+    uint32_t refill_min = o.refill ? std::min<uint32_t>(std::max<uint32_t>(o.refill, 1), 64) : 16;
+    fill_u32_kernel<<<(kQueues * kQueueStride + 255) / 256, 256, 0, s>>>(queue, kQueues * kQueueStride, 0u);
     if (o.counters)
-        cast_persistent_kernel<true><<<blocks, 256, 0, s>>>(w, r, h, max_iters, chunk, refill_min, queue, o.counters);
+        cast_persistent_kernel<true><<<blocks, 256, 0, s>>>(w, r, h, max_iters, chunk, refill_min, cq, o.counters);
     else
-        cast_persistent_kernel<false><<<blocks, 256, 0, s>>>(w, r, h, max_iters, chunk, refill_min, queue, nullptr);
+        cast_persistent_kernel<false><<<blocks, 256, 0, s>>>(w, r, h, max_iters, chunk, refill_min, cq, nullptr);
     VRT_HIP(hipGetLastError());
     return VRT_OK;
 }
@@ -300,7 +367,7 @@ __global__ void cast_one_f64_kernel(WorldView w, double ox, double oy, double oz
         uint32_t si = 0;
         if (sx < nxz && sz < nxz && sy < ny) {
             si = sx | sz << w.shift_xz | sy << (2 * w.shift_xz);
-            m = w.mask[si];
+            m = (uint64_t)w.sec[si].x | (uint64_t)w.sec[si].y << 32;
         }
         if (m == 0) {
             k = 5;
@@ -309,7 +376,7 @@ __global__ void cast_one_f64_kernel(WorldView w, double ox, double oy, double oz
             if (!((m >> bi) & 1ull)) {
                 k = 3;
             } else {
-                uint32_t slot = w.base[si] + (uint32_t)__popcll(m & ((1ull << bi) - 1ull));
+                uint32_t slot = w.sec[si].z + (uint32_t)__popcll(m & ((1ull << bi) - 1ull));
                 uint32_t vi = ((uint32_t)p[0] & 7u) | ((uint32_t)p[2] & 7u) << 3 | ((uint32_t)p[1] & 7u) << 6;
                 k = w.vox[(size_t)slot * 512 + vi] == 0 ? 0 : -1;
             }

@@ -1,0 +1,107 @@
+// torch_render.cpp - libtorch autograd Function around vrt_render_fwd / vrt_render_bwd.
+//
+// The Python module _vrt_torch exposes render(albedo, world, cameras, ...),
+// differentiable in `albedo` (fp32 [num_voxel_slots * 512, 3] on the device,
+// voxel id = slot * 512 + voxel index in the brick).  Forward traces one ray
+// per pixel and shades albedo * max(0, N.L) * intensity; backward scatter-adds
+// dL/dpixel x weight into the albedo gradient.  All device work goes through
+// the C ABI of libvoxelrt_hip.so; torch only owns the tensors.
+#include <torch/extension.h>
+
+#include <cstring>
+#include <stdexcept>
+#include <vector>
+
+#include "vrt.h"
+
+namespace {
+
+void check(int status, const char* what) {
+    if (status != VRT_OK) throw std::runtime_error(std::string(what) + ": " + vrt_last_error());
+}
+
+std::vector<vrt_camera> unpack_cameras(const torch::Tensor& cams) {
+    TORCH_CHECK(cams.device().is_cpu() && cams.scalar_type() == torch::kUInt8 && cams.is_contiguous(),
+                "cameras: contiguous uint8 CPU tensor of packed vrt_camera structs");
+    TORCH_CHECK(cams.numel() % sizeof(vrt_camera) == 0, "cameras: size is not a multiple of sizeof(vrt_camera)");
+    std::vector<vrt_camera> out(cams.numel() / sizeof(vrt_camera));
+    std::memcpy(out.data(), cams.data_ptr(), cams.numel());
+    return out;
+}
+
+class RenderFunction : public torch::autograd::Function<RenderFunction> {
+public:
+    static torch::Tensor forward(torch::autograd::AutogradContext* ctx, torch::Tensor albedo, int64_t world,
+                                 torch::Tensor cameras, int64_t max_iters, std::vector<double> light,
+                                 double intensity, int64_t stream) {
+        TORCH_CHECK(albedo.is_cuda() && albedo.scalar_type() == torch::kFloat32 && albedo.dim() == 2 &&
+                        albedo.size(1) == 3,
+                    "albedo: float32 [voxels, 3] on the GPU");
+        TORCH_CHECK(light.size() == 3, "light: 3 components");
+        auto cams = unpack_cameras(cameras);
+        int64_t npix = 0;
+        for (const auto& c : cams) npix += (int64_t)c.width * c.height;
+        auto alb = albedo.contiguous();
+        auto image = torch::empty({npix, 3}, alb.options());
+        auto saved_voxel = torch::empty({npix}, alb.options().dtype(torch::kInt32));
+        auto saved_weight = torch::empty({npix}, alb.options());
+        float l[3] = {(float)light[0], (float)light[1], (float)light[2]};
+        check(vrt_render_fwd((const vrt_world*)world, cams.data(), (uint32_t)cams.size(), (uint32_t)max_iters, l,
+                             (float)intensity, alb.data_ptr<float>(), image.data_ptr<float>(),
+                             (uint32_t*)saved_voxel.data_ptr<int32_t>(), saved_weight.data_ptr<float>(),
+                             (void*)stream),
+              "vrt_render_fwd");
+        ctx->save_for_backward({saved_voxel, saved_weight});
+        ctx->saved_data["world"] = world;
+        ctx->saved_data["stream"] = stream;
+        ctx->saved_data["nvox"] = albedo.size(0);
+        return image;
+    }
+
+    static torch::autograd::variable_list backward(torch::autograd::AutogradContext* ctx,
+                                                   torch::autograd::variable_list grad_outputs) {
+        auto saved = ctx->get_saved_variables();
+        auto saved_voxel = saved[0], saved_weight = saved[1];
+        int64_t world = ctx->saved_data["world"].toInt();
+        int64_t stream = ctx->saved_data["stream"].toInt();
+        int64_t nvox = ctx->saved_data["nvox"].toInt();
+        auto g = grad_outputs[0].contiguous();
+        auto grad = torch::zeros({nvox, 3}, g.options());
+        check(vrt_render_bwd((const vrt_world*)world, (uint32_t)saved_voxel.numel(), g.data_ptr<float>(),
+                             (const uint32_t*)saved_voxel.data_ptr<int32_t>(), saved_weight.data_ptr<float>(),
+                             grad.data_ptr<float>(), (void*)stream),
+              "vrt_render_bwd");
+        return {grad, torch::Tensor(), torch::Tensor(), torch::Tensor(), torch::Tensor(), torch::Tensor(),
+                torch::Tensor()};
+    }
+};
+
+torch::Tensor render(torch::Tensor albedo, int64_t world, torch::Tensor cameras, int64_t max_iters,
+                     std::vector<double> light, double intensity, int64_t stream) {
+    return RenderFunction::apply(albedo, world, cameras, max_iters, light, intensity, stream);
+}
+
+// Forward only, also returning the saved hit voxel ids and shading weights.
+std::vector<torch::Tensor> render_saved(torch::Tensor albedo, int64_t world, torch::Tensor cameras, int64_t max_iters,
+                                        std::vector<double> light, double intensity, int64_t stream) {
+    auto cams = unpack_cameras(cameras);
+    int64_t npix = 0;
+    for (const auto& c : cams) npix += (int64_t)c.width * c.height;
+    auto alb = albedo.contiguous();
+    auto image = torch::empty({npix, 3}, alb.options());
+    auto sv = torch::empty({npix}, alb.options().dtype(torch::kInt32));
+    auto sw = torch::empty({npix}, alb.options());
+    float l[3] = {(float)light[0], (float)light[1], (float)light[2]};
+    check(vrt_render_fwd((const vrt_world*)world, cams.data(), (uint32_t)cams.size(), (uint32_t)max_iters, l,
+                         (float)intensity, alb.data_ptr<float>(), image.data_ptr<float>(),
+                         (uint32_t*)sv.data_ptr<int32_t>(), sw.data_ptr<float>(), (void*)stream),
+          "vrt_render_fwd");
+    return {image, sv, sw};
+}
+
+}  // namespace
+
+PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
+    m.def("render", &render, "differentiable Lambertian render (libtorch autograd Function)");
+    m.def("render_saved", &render_saved, "forward render returning image, saved voxel ids and weights");
+}

@@ -5,7 +5,21 @@
 // fmaf for sideDist and the next position, x86 min and floor2i semantics,
 // +0.001 bias, 2-level LOD from the 64-bit masks (brick level 8/16/32 voxels,
 // voxel level 1/2/4).  Storage is the popcount-compacted GPU layout
-// (GpuRenderer.cpp:27-33, VoxelMap.glsl:45-64) instead of the CPU's dense one.
+// (GpuRenderer.cpp:27-33, VoxelMap.glsl:45-64) with the sector's mask and
+// base slot packed in one 16-byte record, so a lookup is one load.
+//
+// Two instantiations of the step compute identical traversals:
+//   EXACT = true   floor2i with x86 cvtps2dq semantics (NaN / out of range ->
+//                  INT_MIN) and x86 minps (a < b ? a : b) in the loop;
+//   EXACT = false  plain floor + convert and v_min3_f32, valid for "tame" rays
+//                  (finite 1/d and tStart, |o| < 2^20, 0.25 <= |d|^2 <= 4):
+//                  their positions stay far inside int32 range for any
+//                  iteration cap (a step moves at most 32*sqrt(3)/|d| + 0.001
+//                  along d, i.e. < 56 voxels, and 65535 * 56 + 2^20 < 2^31),
+//                  and their side distances are never NaN, so floor2i never
+//                  takes its INT_MIN branch and the two minimums agree up to
+//                  the sign of a zero, which cannot change tmin + 0.001.
+// The reported t is recomputed with the exact x86 minimum in both cases.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -18,6 +32,7 @@ namespace vrt {
 struct RayState {
     float o[3], d[3], inv[3], ts[3], side[3], cur[3];
     int32_t W[3];
+    int32_t dm[3];  // all ones where d >= 0: the cell corner to step to is the far one
     uint32_t iters, descents;
 };
 
@@ -34,51 +49,76 @@ __device__ __forceinline__ void ray_setup(RayState& r, float ox, float oy, float
         r.ts[a] = ((r.d[a] < 0.0f ? 0.0f : 1.0f) - r.o[a]) * r.inv[a];
         r.side[a] = 0.0f;
         r.cur[a] = r.o[a];
+        r.dm[a] = r.d[a] < 0.0f ? 0 : -1;
     }
     r.iters = 0;
     r.descents = 0;
 }
 
+// True when the fast step is exact for this ray (see the file comment).
+__device__ __forceinline__ bool ray_is_tame(const RayState& r) {
+    bool ok = true;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        ok = ok && fabsf(r.inv[a]) < INFINITY && fabsf(r.ts[a]) < INFINITY && fabsf(r.o[a]) < 1048576.0f;
+    }
+    float len2 = r.d[0] * r.d[0] + r.d[1] * r.d[1] + r.d[2] * r.d[2];
+    return ok && len2 >= 0.25f && len2 <= 4.0f;
+}
+
 // One iteration of RayCast (CpuRenderer.cpp:185-202) with GetStepPos (:135-171).
 // On STEP_HIT, v is the hit voxel and *hit_slot its brick slot.
+template <bool EXACT>
 __device__ __forceinline__ int ray_step(const WorldView& w, RayState& r, int32_t v[3], uint32_t* hit_slot) {
 #pragma unroll
-    for (int a = 0; a < 3; a++) v[a] = (int32_t)((uint32_t)r.W[a] + (uint32_t)floor2i(r.cur[a]));
+    for (int a = 0; a < 3; a++) {
+        int32_t f = EXACT ? floor2i(r.cur[a]) : (int32_t)floorf(r.cur[a]);
+        v[a] = (int32_t)((uint32_t)r.W[a] + (uint32_t)f);
+    }
     // GetInboundMask (:114-117)
     if (!((uint32_t)(v[0] | v[2]) < w.size_xz && (uint32_t)v[1] < w.size_y)) return STEP_MISS;
     r.iters++;
     uint32_t ux = (uint32_t)v[0], uy = (uint32_t)v[1], uz = (uint32_t)v[2];
     uint32_t si = (ux >> 5) | (uz >> 5) << w.shift_xz | (uy >> 5) << (2 * w.shift_xz);
-    uint64_t m = w.mask[si];
-    uint32_t bi = ((ux >> 3) & 3u) | ((uz >> 3) & 3u) << 2 | ((uy >> 3) & 3u) << 4;
-    uint32_t half = bi < 32 ? (uint32_t)m : (uint32_t)(m >> 32);
-    uint32_t lod = 3;
-    if ((half >> (bi & 31u)) & 1u) {
+    uint4 sec = w.sec[si];
+    // keep the base slot in the same 16-byte load as the mask: a descent then waits on
+    // one load (the occupancy mask) instead of two
+    asm volatile("" : "+v"(sec.z));
+    uint64_t m = (uint64_t)sec.x | (uint64_t)sec.y << 32;
+    uint32_t bi = __builtin_amdgcn_ubfe(ux, 3, 2) | __builtin_amdgcn_ubfe(uz, 3, 2) << 2 |
+                  __builtin_amdgcn_ubfe(uy, 3, 2) << 4;
+    uint32_t unit = 8;  // cell edge at the level below the mask: a brick
+    if ((uint32_t)(m >> bi) & 1u) {
         r.descents++;
-        uint32_t slot = w.base[si] + (uint32_t)__popcll(m & ((1ull << bi) - 1ull));
-        uint32_t cell = ((ux >> 2) & 1u) | ((uz >> 2) & 1u) << 1 | ((uy >> 2) & 1u) << 2;
+        uint32_t slot = sec.z + (uint32_t)__popcll(m & ((1ull << bi) - 1ull));
+        uint32_t cell = __builtin_amdgcn_ubfe(ux, 2, 1) | __builtin_amdgcn_ubfe(uz, 2, 1) << 1 |
+                        __builtin_amdgcn_ubfe(uy, 2, 1) << 2;
         m = w.occ[(size_t)slot * 8 + cell];
         bi = (ux & 3u) | (uz & 3u) << 2 | (uy & 3u) << 4;
-        half = bi < 32 ? (uint32_t)m : (uint32_t)(m >> 32);
-        lod = 0;
-        if ((half >> (bi & 31u)) & 1u) {
+        unit = 1;
+        if ((uint32_t)(m >> bi) & 1u) {
             *hit_slot = slot;
             return STEP_HIT;
         }
     }
-    // level4: the 4x4x4 mask is empty; level2: the 2x2x2 group of bi is empty
-    if (m == 0)
-        lod += 2;
-    else if (((half >> (bi & 0xAu)) & 0x00330033u) == 0)
-        lod += 1;
-    int32_t cm = (int32_t)((1u << lod) - 1u);
+    // level4: the whole 4x4x4 mask is empty; level2: the 2x2x2 group holding bi is empty
+    // ((half >> (bi & 0xA)) & 0x00330033 on the 32-bit half selected by bit 5 of bi)
+    uint32_t grp = (uint32_t)(m >> (bi & 0x2Au)) & 0x00330033u;
+    uint32_t k = m == 0 ? 2u : (grp == 0 ? 1u : 0u);
+    int32_t cm = (int32_t)((unit << k) - 1u);
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-        v[a] = r.d[a] < 0.0f ? (v[a] & ~cm) : (v[a] | cm);
+        // d < 0: v & ~cm (near corner); d >= 0: v | cm (far corner)
+        v[a] = (v[a] & ~cm) | (cm & r.dm[a]);
         int32_t rel = (int32_t)((uint32_t)v[a] - (uint32_t)r.W[a]);
         r.side[a] = fmaf((float)rel, r.inv[a], r.ts[a]);
     }
-    float tmin = min_x86(min_x86(r.side[0], r.side[1]), r.side[2]) + 0.001f;
+    float tmin;
+    if (EXACT)
+        tmin = min_x86(min_x86(r.side[0], r.side[1]), r.side[2]);
+    else
+        tmin = fminf(fminf(r.side[0], r.side[1]), r.side[2]);
+    tmin += 0.001f;
 #pragma unroll
     for (int a = 0; a < 3; a++) r.cur[a] = fmaf(tmin, r.d[a], r.o[a]);
     return STEP_CONTINUE;

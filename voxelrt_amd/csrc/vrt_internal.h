@@ -72,9 +72,9 @@ private:
 
 // ---------------------------------------------------------------- device world view
 
+// Sector record in HBM: allocation mask (lo, hi), base slot, reserved (vrt_sector_record in vrt.h).
 struct WorldView {
-    const uint64_t* __restrict__ mask;   // [nsec]
-    const uint32_t* __restrict__ base;   // [nsec]
+    const uint4* __restrict__ sec;       // [nsec]
     const uint64_t* __restrict__ occ;    // [slots * 8]
     const uint8_t* __restrict__ vox;     // [slots * 512]
     uint32_t size_xz, size_y;            // voxel extents
@@ -88,8 +88,7 @@ struct vrt_world {
     uint32_t log2_xz = 0, log2_y = 0;
     uint32_t shift_xz = 0, shift_y = 0;
     uint32_t nsec = 0;
-    uint64_t* d_mask = nullptr;
-    uint32_t* d_base = nullptr;
+    uint4* d_sec = nullptr;           // packed sector records (mask, base slot)
     uint64_t* d_occ = nullptr;
     uint8_t* d_vox = nullptr;
     uint64_t* d_palette = nullptr;
@@ -101,8 +100,7 @@ struct vrt_world {
 
     vrt::WorldView view() const {
         vrt::WorldView v;
-        v.mask = d_mask;
-        v.base = d_base;
+        v.sec = d_sec;
         v.occ = d_occ;
         v.vox = d_vox;
         v.size_xz = 1u << log2_xz;

@@ -171,8 +171,7 @@ static int grow_slots(vrt_world* w, uint32_t need, hipStream_t s) {
 }
 
 static void free_world(vrt_world* w) {
-    (void)hipFree(w->d_mask);
-    (void)hipFree(w->d_base);
+    (void)hipFree(w->d_sec);
     (void)hipFree(w->d_occ);
     (void)hipFree(w->d_vox);
     (void)hipFree(w->d_palette);
@@ -214,12 +213,10 @@ int vrt_world_create(const vrt_world_desc* desc, void* stream, vrt_world** out) 
         auto check = [&](hipError_t e, const char* what) {
             if (e != hipSuccess && rc == VRT_OK) rc = hip_fail(e, what);
         };
-        check(hipMalloc(&w->d_mask, (size_t)w->nsec * 8), "hipMalloc(sector masks)");
-        check(hipMalloc(&w->d_base, (size_t)w->nsec * 4), "hipMalloc(base slots)");
+        check(hipMalloc(&w->d_sec, (size_t)w->nsec * 16), "hipMalloc(sector records)");
         check(hipMalloc(&w->d_palette, 256 * 8), "hipMalloc(palette)");
         if (rc == VRT_OK) {
-            check(hipMemsetAsync(w->d_mask, 0, (size_t)w->nsec * 8, s), "hipMemsetAsync(sector masks)");
-            check(hipMemsetAsync(w->d_base, 0, (size_t)w->nsec * 4, s), "hipMemsetAsync(base slots)");
+            check(hipMemsetAsync(w->d_sec, 0, (size_t)w->nsec * 16, s), "hipMemsetAsync(sector records)");
             check(hipMemsetAsync(w->d_palette, 0, 256 * 8, s), "hipMemsetAsync(palette)");
         }
         if (rc == VRT_OK && desc->initial_slots) rc = grow_slots(w, std::min(desc->initial_slots, w->arena.capacity()), s);
@@ -252,7 +249,7 @@ int vrt_world_get_info(const vrt_world* w, vrt_world_info* out) {
         out->num_sectors = w->nsec;
         out->num_bricks = w->num_bricks;
         out->slot_capacity = w->slot_cap;
-        out->device_bytes = (uint64_t)w->nsec * 12 + 256 * 8 + (uint64_t)w->slot_cap * (512 + 64);
+        out->device_bytes = (uint64_t)w->nsec * 16 + 256 * 8 + (uint64_t)w->slot_cap * (512 + 64);
         return VRT_OK;
     });
 }
@@ -260,8 +257,7 @@ int vrt_world_get_info(const vrt_world* w, vrt_world_info* out) {
 int vrt_world_get_buffers(const vrt_world* w, vrt_world_buffers* out) {
     return guarded([&]() -> int {
         if (!w || !out) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_world_get_buffers: null argument");
-        out->sector_mask = w->d_mask;
-        out->base_slot = w->d_base;
+        out->sectors = (const vrt_sector_record*)w->d_sec;
         out->occupancy = w->d_occ;
         out->voxels = w->d_vox;
         out->palette = w->d_palette;
@@ -359,8 +355,10 @@ int vrt_world_sync(vrt_world* w, uint32_t nsec, const uint32_t* sector_idx, cons
                 return hip_fail(e, "occupancy build");
             }
         }
-        VRT_HIP(hipMemcpyAsync(w->d_mask, w->h_mask.data(), (size_t)w->nsec * 8, hipMemcpyHostToDevice, s));
-        VRT_HIP(hipMemcpyAsync(w->d_base, w->h_base.data(), (size_t)w->nsec * 4, hipMemcpyHostToDevice, s));
+        std::vector<uint4> recs(w->nsec);
+        for (uint32_t i = 0; i < w->nsec; i++)
+            recs[i] = make_uint4((uint32_t)w->h_mask[i], (uint32_t)(w->h_mask[i] >> 32), w->h_base[i], 0u);
+        VRT_HIP(hipMemcpyAsync(w->d_sec, recs.data(), (size_t)w->nsec * 16, hipMemcpyHostToDevice, s));
         hipError_t e = hipStreamSynchronize(s);
         (void)hipFree(d_slots);
         if (e != hipSuccess) return hip_fail(e, "vrt_world_sync: hipStreamSynchronize");
