@@ -49,6 +49,159 @@ def algorithmic_bytes(c, with_pos: bool) -> float:
     return rays * (B_RAY_IN + B_HIT_OUT + (B_POS_OUT if with_pos else 0)) + iters * B_ITER + desc * B_DESCENT + hits * B_HIT
 
 
+def _time_ms(fn, reps: int) -> float:
+    import torch
+
+    fn()
+    torch.cuda.synchronize()
+    s = torch.cuda.Event(enable_timing=True)
+    e = torch.cuda.Event(enable_timing=True)
+    s.record()
+    for _ in range(reps):
+        fn()
+    e.record()
+    torch.cuda.synchronize()
+    return s.elapsed_time(e) / reps
+
+
+def bench_batched(world, max_iters: int, dev) -> dict:
+    """64 cameras of uneven square sizes (128..1024 px) on the same world plus each camera's
+    bounce set truncated to a seeded fraction U(0.25, 1): all 128 segments in one hipGraph
+    launch with the autotuned tuning, against plain vrt_cast launches over the same rays."""
+    import numpy as np
+    import torch
+
+    import voxelrt_amd as V
+    from voxelrt_amd import workloads as W
+
+    log("batched: 64 cameras + bounce sets")
+    cams = W.c4_cameras(64, seed=4, tile=(1, 1))
+    rng = np.random.default_rng(44)
+    prim = [c.rays_device() for c in cams]
+    parts, segs = [], []
+    off = 0
+    for c, rb in zip(cams, prim):
+        h = world.cast(rb, max_iters, with_pos=True)
+        br, _, cnt = V.bounce(rb, h, c.c.width, W.BOUNCE_SEED)
+        torch.cuda.synchronize()
+        nb = int(int(cnt.item()) * rng.uniform(0.25, 1.0))
+        parts.append((br.o[:, :nb].clone(), br.d[:, :nb].clone()))
+        segs.append((off, nb, rb.world_origin, None))
+        off += nb
+        del h, br
+    rays = V.RayBatch(torch.cat([p[0] for p in parts], 1).contiguous(),
+                      torch.cat([p[1] for p in parts], 1).contiguous(), (0, 0, 0))
+    del parts
+    batch = V.Batch(cams, segs, rays, max_iters)
+    tuning, tune_ms = V.autotune(world, batch, None)
+    plan = V.BatchPlan(world, batch, tuning)
+    n_prim = sum(c.n for c in cams)
+    n_total = batch.total
+    ms_graph = _time_ms(plan.graph_launch, 10)
+    ms_launch = _time_ms(plan.launch, 10)
+    # plain per-ray kernel on the same rays: 128 launches over materialised buffers, camera rays
+    # materialised in the tuned tile order (the plain kernel's best case)
+    tiled = [V.CameraSpec((0, 0, 0), 0, 0, 16, 16) for _ in cams]
+    for t_cam, c in zip(tiled, cams):
+        t_cam.c = type(c.c).from_buffer_copy(c.c)
+        t_cam.c.tile_w, t_cam.c.tile_h = tuning[0], tuning[1]
+    prim_tiled = [c.rays_device() for c in tiled]
+    del prim
+    bounce_views = [V.RayBatch(rays.o[:, ro:ro + n].contiguous(), rays.d[:, ro:ro + n].contiguous(), Wo)
+                    for (ro, n, Wo, _) in segs]
+    all_rays = prim_tiled + bounce_views
+    outs = [V.HitBatch.empty(r.n) for r in all_rays]
+
+    def plain():
+        for r, o in zip(all_rays, outs):
+            world.cast(r, max_iters, out=o)
+
+    ms_plain = _time_ms(plain, 5)
+    # parity of the batch against the plain launches: bounce segments element-wise, camera
+    # segments as hit multisets (the plain buffers are in tile order)
+    plan.launch()
+    torch.cuda.synchronize()
+    ok = all(torch.equal(batch.segment_hits(len(cams) + k)[1], outs[len(cams) + k].rec) for k in range(len(segs)))
+    for k in range(len(cams)):
+        a = batch.segment_hits(k)[1]
+        b = outs[k].rec
+        ka = (a[:, 0].long() << 40) ^ (a[:, 1].long() << 20) ^ a[:, 2].long() ^ (a[:, 3].long() << 8)
+        kb = (b[:, 0].long() << 40) ^ (b[:, 1].long() << 20) ^ b[:, 2].long() ^ (b[:, 3].long() << 8)
+        ok = ok and torch.equal(torch.sort(ka).values, torch.sort(kb).values)
+    res = {
+        "value": round(n_total / ms_graph / 1e3, 1),
+        "unit": "Mrays/s",
+        "rays": n_total,
+        "primary_rays": n_prim,
+        "segments": len(cams) + len(segs),
+        "ms_per_launch": round(ms_graph, 4),
+        "ms_per_launch_no_graph": round(ms_launch, 4),
+        "tuning": {"tile": f"{tuning[0]}x{tuning[1]}", "rays_per_lane": tuning[2], "chunk": tuning[3]},
+        "plain_value": round(n_total / ms_plain / 1e3, 1),
+        "plain_launches": len(outs),
+        "parity_with_plain": bool(ok),
+    }
+    log(f"  batched {res['value']} Mrays/s ({n_total} rays, tuning {tuning}), plain {res['plain_value']} Mrays/s, "
+        f"parity {ok}")
+    plan.close()
+    return res
+
+
+def bench_differentiable(dev) -> dict:
+    """512^3 terrain, per-voxel RGB albedo U[0,1], 16 views of 512x512, L2 loss against images
+    rendered from albedo + N(0, 0.05): forward + backward through the libtorch autograd Function."""
+    import torch
+
+    import voxelrt_amd as V
+    from voxelrt_amd import workloads as W
+
+    log("differentiable: 512^3 terrain, 16 views of 512x512")
+    scene = W.C5_WORLD.build()
+    world = V.World.from_scene(scene)
+    cams = W.c5_cameras(16, 512)
+    nvox = world.info().num_bricks * 512
+    g = torch.Generator(device=dev).manual_seed(5)
+    albedo = torch.rand((nvox, 3), generator=g, device=dev, dtype=torch.float32)
+    target, _, _ = V.render_saved(world, (albedo + 0.05 * torch.randn(albedo.shape, generator=g, device=dev)),
+                                  cams, W.MAX_ITERS)
+    npix = sum(c.n for c in cams)
+    a = albedo.clone().requires_grad_(True)
+
+    def step():
+        a.grad = None
+        img = V.render(world, a, cams, W.MAX_ITERS)
+        loss = ((img - target) ** 2).sum()
+        loss.backward()
+
+    ms_step = _time_ms(step, 10)
+    ms_fwd = _time_ms(lambda: V.render_saved(world, albedo, cams, W.MAX_ITERS), 10)
+    img, sv, sw = V.render_saved(world, albedo, cams, W.MAX_ITERS)
+    gi = 2.0 * (img - target)
+    ms_bwd = _time_ms(lambda: V.render_backward(world, gi, sv, sw, nvox), 10)
+    prims = [c.rays_device() for c in cams]
+    outs = [V.HitBatch.empty(r.n) for r in prims]
+
+    def plain():
+        for r, o in zip(prims, outs):
+            world.cast(r, W.MAX_ITERS, out=o)
+
+    ms_plain = _time_ms(plain, 10)
+    res = {
+        "value": round(npix / ms_step / 1e3, 1),
+        "unit": "Mrays/s",
+        "what": "forward + backward (autograd Function, loss and its gradient included)",
+        "rays": npix,
+        "ms_per_step": round(ms_step, 4),
+        "forward_only_value": round(npix / ms_fwd / 1e3, 1),
+        "backward_kernel_ms": round(ms_bwd, 4),
+        "plain_value": round(npix / ms_plain / 1e3, 1),
+        "voxels": nvox,
+    }
+    log(f"  fwd+bwd {res['value']} Mrays/s, forward {res['forward_only_value']}, plain {res['plain_value']} Mrays/s")
+    del world
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -60,6 +213,8 @@ def main() -> int:
     ap.add_argument("--refill", type=int, default=0)
     ap.add_argument("--cpu-runs", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-diff", action="store_true")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="also run one device copy of this many MiB (a known-size kernel for scaling PMC byte counters)")
     args = ap.parse_args()
@@ -236,6 +391,9 @@ def main() -> int:
         }
         log(f"CPU {cpu['value']} Mrays/s on {cores} cores; GPU/CPU parity {ok}")
 
+    batched = None if args.no_batched else bench_batched(world, W.MAX_ITERS, dev)
+    differentiable = None if args.no_diff else bench_differentiable(dev)
+
     line = {
         "metric": "Mrays/s (1024^3, primary+diffuse)",
         "value": round(value, 1),
@@ -261,6 +419,8 @@ def main() -> int:
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "batched": batched,
+        "differentiable": differentiable,
     }
     print(json.dumps(line), flush=True)
     return 0

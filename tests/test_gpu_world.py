@@ -1,0 +1,177 @@
+"""GPU checks of the device world: incremental sector sync (the DirtyLocs protocol of
+GpuRenderer.cpp:45-167), slot exhaustion, error codes and forced HIP failures."""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from helpers import assert_hits_equal, blob_world, random_rays
+
+pytestmark = pytest.mark.gpu
+
+
+def sector_stream_of(vox: np.ndarray, sectors: list[int]):
+    """Sector stream (idx, mask, bricks) of the listed sector indices of a dense [Y][Z][X] grid."""
+    sy, sz, sx = vox.shape
+    nxz = sx // 32
+    sh = nxz.bit_length() - 1
+    idx, masks, bricks = [], [], []
+    for si in sectors:
+        cx, cz, cy = si & (nxz - 1), (si >> sh) & (nxz - 1), si >> (2 * sh)
+        sec = vox[cy * 32:(cy + 1) * 32, cz * 32:(cz + 1) * 32, cx * 32:(cx + 1) * 32]
+        m = 0
+        for b in range(64):
+            bx, bz, by = b & 3, (b >> 2) & 3, b >> 4
+            br = sec[by * 8:(by + 1) * 8, bz * 8:(bz + 1) * 8, bx * 8:(bx + 1) * 8]
+            if br.any():
+                m |= 1 << b
+                bricks.append(br.reshape(-1))
+        idx.append(si)
+        masks.append(m)
+    br = np.concatenate(bricks) if bricks else np.zeros(0, np.uint8)
+    return np.array(idx, np.uint32), np.array(masks, np.uint64), br
+
+
+def cast_np(world, o, d, W=(0, 0, 0), max_iters=128):
+    from voxelrt_amd import RayBatch
+
+    h = world.cast(RayBatch.from_numpy(o, d, W), max_iters)
+    torch.cuda.synchronize()
+    return h.t.cpu().numpy(), h.rec.cpu().numpy()
+
+
+def test_incremental_sync_equals_fresh_world(vrt, oracle_mod, cuda):
+    import voxelrt_amd as V
+    from test_oracle import dense_world
+
+    rng = np.random.default_rng(3)
+    vox = blob_world(128, seed=21, density=0.01)
+    world = V.World(7, 7)
+    nsec = 4 ** 3
+    world.sync(*sector_stream_of(vox, list(range(nsec))))
+    o, d = random_rays(20000, 128, seed=5)
+    for step in range(4):
+        # edit: clear a box, paint another, empty a whole sector
+        v2 = vox.copy()
+        a = rng.integers(0, 100, 3)
+        v2[a[1]:a[1] + 20, a[2]:a[2] + 20, a[0]:a[0] + 20] = 0
+        b = rng.integers(0, 110, 3)
+        v2[b[1]:b[1] + 12, b[2]:b[2] + 12, b[0]:b[0] + 12] = rng.integers(1, 256)
+        c = rng.integers(0, 4, 3)
+        v2[c[1] * 32:(c[1] + 1) * 32, c[2] * 32:(c[2] + 1) * 32, c[0] * 32:(c[0] + 1) * 32] = 0
+        changed = []
+        for si in range(nsec):
+            cx, cz, cy = si & 3, (si >> 2) & 3, si >> 4
+            sl = np.s_[cy * 32:(cy + 1) * 32, cz * 32:(cz + 1) * 32, cx * 32:(cx + 1) * 32]
+            if not np.array_equal(vox[sl], v2[sl]):
+                changed.append(si)
+        rng.shuffle(changed)
+        world.sync(*sector_stream_of(v2, changed))
+        vox = v2
+        ref = dense_world(oracle_mod, vox).cast(o, d, (0, 0, 0), 128)
+        t, rec = cast_np(world, o, d)
+        assert_hits_equal(t, rec, ref["t"], ref["rec"], f"edit {step}")
+        fresh = V.World.from_scene(V.Scene.from_dense(vox))
+        t2, rec2 = cast_np(fresh, o, d)
+        assert np.array_equal(rec, rec2)
+        info = world.info()
+        assert info.num_bricks == sum(bin(int(m)).count("1") for m in sector_stream_of(vox, list(range(nsec)))[1])
+
+
+def test_slot_exhaustion_raises_and_keeps_world(vrt, cuda):
+    import voxelrt_amd as V
+
+    vox = blob_world(64, seed=8, density=0.0, blobs=3)
+    idx, masks, bricks = sector_stream_of(vox, list(range(8)))
+    need = sum(bin(int(m)).count("1") for m in masks)
+    assert 0 < need < 512
+    world = V.World(6, 6, max_slots=need)
+    world.sync(idx, masks, bricks)
+    o, d = random_rays(4000, 64, seed=2)
+    before = cast_np(world, o, d)
+    full = np.full((64, 64, 64), 3, np.uint8)
+    with pytest.raises(V.CapacityError, match="Could not allocate brick slots"):
+        world.sync(*sector_stream_of(full, list(range(8))))
+    after = cast_np(world, o, d)
+    assert np.array_equal(before[1], after[1])
+    assert world.info().num_bricks == need
+
+
+def test_error_codes(vrt, cuda):
+    import voxelrt_amd as V
+    from voxelrt_amd import _lib as L
+
+    world = V.World(6, 6)
+    with pytest.raises(V.VrtError) as e:
+        world.sync(np.array([1, 1], np.uint32), np.array([1, 1], np.uint64), np.zeros(1024, np.uint8))
+    assert e.value.status == L.VRT_ERR_INVALID_ARGUMENT and "twice" in str(e.value)
+    with pytest.raises(V.VrtError) as e:
+        world.sync(np.array([9], np.uint32), np.array([1], np.uint64), np.zeros(512, np.uint8))
+    assert "out of range" in str(e.value)
+    o, d = random_rays(64, 64, seed=1)
+    rb = V.RayBatch.from_numpy(o, d)
+    for bad in (0, 70000):
+        with pytest.raises(V.VrtError):
+            world.cast(rb, bad)
+    # a misaligned hit record buffer is rejected
+    h = V.HitBatch.empty(64)
+    raw = torch.empty(64 * 16 + 8, dtype=torch.uint8, device="cuda")
+    hits = L.Hits(h.t.data_ptr(), raw.data_ptr() + 4, 0)
+    r = rb.to_c()
+    import ctypes as C
+
+    assert V.lib().vrt_cast(world.handle, C.byref(r), 128, C.byref(hits), None) == L.VRT_ERR_INVALID_ARGUMENT
+    # zero rays and an empty sync are fine
+    world.cast(V.RayBatch.from_numpy(np.zeros((3, 0), np.float32), np.zeros((3, 0), np.float32)), 128)
+    world.sync(np.zeros(0, np.uint32), np.zeros(0, np.uint64), np.zeros(0, np.uint8))
+
+
+def test_forced_hip_failure_is_reported(vrt, cuda):
+    import voxelrt_amd as V
+    from voxelrt_amd import _lib as L
+
+    world = V.World(6, 6)
+    o, d = random_rays(64, 64, seed=1)
+    rb = V.RayBatch.from_numpy(o, d)
+    os.environ["VRT_FAULT_INJECT"] = "vrt_cast"
+    try:
+        with pytest.raises(V.VrtError) as e:
+            world.cast(rb, 128)
+        assert e.value.status == L.VRT_ERR_HIP and "injected" in str(e.value)
+    finally:
+        del os.environ["VRT_FAULT_INJECT"]
+    world.cast(rb, 128)
+    torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("sparse", [True, False])
+def test_c3_spheres_parity(vrt, oracle_mod, cuda, sparse):
+    """2048^3 spheres (587 sparse / 20k literal, r ~ U(4, 64)): 1920x1080 primary rays plus
+    2M rays with origins uniform in the box and directions uniform on the sphere."""
+    import time
+
+    import voxelrt_amd as V
+    from voxelrt_amd import workloads as W
+
+    scene = W.c3_world(sparse)
+    world = V.World.from_scene(scene)
+    ow = oracle_mod.OracleWorld.from_stream(scene.stream())
+    cam = W.c3_camera()
+    prim = cam.rays_device()
+    o_i, d_i = V.incoherent_rays_host(2_000_000, 2048, 2048, seed=33)
+    inc = V.RayBatch.from_numpy(o_i, d_i)
+    for name, rb in (("primary", prim), ("incoherent", inc)):
+        h = world.cast(rb, W.MAX_ITERS)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(5):
+            world.cast(rb, W.MAX_ITERS, out=h)
+        torch.cuda.synchronize()
+        rate = rb.n * 5 / (time.perf_counter() - t0) / 1e6
+        ref = ow.cast_simd(rb.o.cpu().numpy(), rb.d.cpu().numpy(), rb.world_origin, W.MAX_ITERS)
+        assert_hits_equal(h.t.cpu().numpy(), h.rec.cpu().numpy(), ref["t"], ref["rec"], f"C3 {name}")
+        hit = float(((ref["rec"][:, 3] & 0xFF) != 0).mean())
+        print(f"C3 sparse={sparse} {name}: {rb.n} rays, {rate:.0f} Mrays/s, hit fraction {hit:.3f}", flush=True)

@@ -42,7 +42,12 @@ bool fault_injected(const char* entry) {
 // On failure the arena is left with the old run released; callers work on a
 // copy and discard it when any sector does not fit.
 bool SlotArena::realloc(uint32_t& base, uint32_t cur, uint32_t want) {
-    if (want == cur && cur != 0) return true;
+    if (want == 0) {
+        if (cur != 0) release(base, cur);
+        base = 0;
+        return true;
+    }
+    if (want == cur) return true;
     if (want < cur) {
         release(base + want, cur - want);
         if (want == 0) base = 0;

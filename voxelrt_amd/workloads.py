@@ -67,7 +67,8 @@ def c3_world(sparse: bool, threads: int = 0) -> Scene:
 
 
 def c3_camera(tile=(8, 8)) -> CameraSpec:
-    return CameraSpec((1024.5, 1024.25, -40.75), yaw_deg=3.0, pitch_deg=-2.0, width=1920, height=1080, tile=tile)
+    """Inside the box near its z = 0 face, looking across it."""
+    return CameraSpec((1024.37, 1100.61, 6.29), yaw_deg=3.0, pitch_deg=-4.0, width=1920, height=1080, tile=tile)
 
 
 def c4_cameras(n: int = 64, seed: int = 4, tile=(8, 8)) -> list[CameraSpec]:
