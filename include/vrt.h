@@ -164,7 +164,9 @@ int vrt_cast(const vrt_world* world, const vrt_rays* rays, uint32_t max_iters,
 typedef enum vrt_cast_schedule {
     VRT_SCHED_AUTO = 0,        /* pick from the autotuner cache, else the default */
     VRT_SCHED_GRID = 1,        /* one ray per lane, one lane per ray, grid covers n */
-    VRT_SCHED_PERSISTENT = 2   /* persistent waves pull ray chunks from a queue and refill idle lanes */
+    VRT_SCHED_PERSISTENT = 2,  /* persistent waves pull ray chunks from a queue and refill idle lanes */
+    VRT_SCHED_MULTIPASS = 3    /* passes of pass_iters lookups; rays still running are packed into
+                                  dense waves again for the next pass (long-tailed ray sets) */
 } vrt_cast_schedule;
 
 typedef struct vrt_cast_opts {
@@ -175,13 +177,19 @@ typedef struct vrt_cast_opts {
     vrt_cast_counters* counters;/* optional device pointer, accumulated (not cleared) */
     const uint32_t* count;      /* optional device pointer: rays to trace, read when the kernel
                                    runs (rays->n is then the capacity and the upper bound) */
+    uint32_t pass_iters;        /* multipass: lookups per pass (0 = 16) */
+    uint32_t passes;            /* multipass: number of passes, the last one runs to max_iters (0 = 5) */
+    void* scratch;              /* multipass: device scratch of vrt_cast_scratch_bytes(n) bytes; NULL = a
+                                   per-device buffer owned by the library (multipass casts on one device
+                                   must then not overlap across streams) */
+    size_t scratch_bytes;
 } vrt_cast_opts;
 
 int vrt_cast_ex(const vrt_world* world, const vrt_rays* rays, uint32_t max_iters,
                 vrt_hits* hits, const vrt_cast_opts* opts, void* stream);
 
-/* Bytes of device scratch a persistent cast needs (queue counters). */
-size_t vrt_cast_scratch_bytes(void);
+/* Bytes of device scratch a multipass cast of n rays needs (continuation records and counters). */
+size_t vrt_cast_scratch_bytes(uint32_t n);
 
 /*
  * Scalar FP64 cast of one ray on the device, the semantics of VoxelMap::RayCast
@@ -242,6 +250,31 @@ typedef struct vrt_camera {
 
 /* Generate the primary rays of a camera into device buffers (n = width*height). */
 int vrt_camera_rays(const vrt_camera* cam, vrt_ray_buffers* out, void* stream);
+
+/* ---------------------------------------------------------------- fused frame */
+
+/*
+ * Primary ray plus one diffuse bounce per hit, traced per pixel in one kernel: the structure of
+ * the reference GPU renderer (Shaders/VoxelRender.comp:28-94, one invocation per pixel traces
+ * the primary ray, then its bounce), without shading.  `primary` receives exactly what vrt_cast
+ * returns for the camera's rays (vrt_camera_rays); `bounce` is indexed by pixel: entry i holds
+ * the hit of the bounce ray vrt_bounce_generate builds for primary hit i (width = camera width,
+ * `seed`), or, when primary i missed, a record with info = VRT_HIT_NO_RAY and t = -1.
+ * primary->pos is optional.  Schedules: VRT_SCHED_GRID (one lane per pixel) or
+ * VRT_SCHED_PERSISTENT (waves refill lanes in batches of `refill`: primary rays are built from
+ * the camera in registers, so a refill waits on no load).
+ */
+#define VRT_HIT_NO_RAY (1u << 14)
+
+typedef struct vrt_frame_opts {
+    uint32_t schedule;            /* VRT_SCHED_GRID (default) or VRT_SCHED_PERSISTENT */
+    uint32_t refill;              /* persistent: parked lanes that trigger a retire/refill (0 = 16) */
+    uint32_t chunk;               /* persistent: pixels per queue grab, multiple of 64 (0 = 256) */
+    vrt_cast_counters* counters;  /* optional device totals over primary and bounce rays */
+} vrt_frame_opts;
+
+int vrt_cast_frame(const vrt_world* world, const vrt_camera* cam, uint32_t max_iters, uint32_t seed,
+                   vrt_hits* primary, vrt_hits* bounce, const vrt_frame_opts* opts, void* stream);
 
 /* ---------------------------------------------------------------- batched */
 

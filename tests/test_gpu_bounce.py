@@ -13,7 +13,7 @@ pytestmark = pytest.mark.gpu
 
 
 def run_pipeline(V, world, prim, width, seed, max_iters, schedule=1):
-    hits = world.cast(prim, max_iters, with_pos=True, schedule=schedule)
+    hits = world.cast(prim, max_iters, with_pos=True, schedule=1)
     br, src, count = V.bounce(prim, hits, width, seed)
     hb = world.cast(br, max_iters, schedule=schedule, count=count)
     torch.cuda.synchronize()
@@ -33,7 +33,7 @@ def test_primary_and_bounce_parity(vrt, oracle_mod, cuda, config):
     world = V.World.from_scene(scene)
     ow = oracle_mod.OracleWorld.from_stream(scene.stream())
     prim = cam.rays_device()
-    hits, br, src, nb, hb = run_pipeline(V, world, prim, cam.c.width, W.BOUNCE_SEED, W.MAX_ITERS)
+    hits, br, src, nb, hb = run_pipeline(V, world, prim, cam.c.width, W.BOUNCE_SEED, W.MAX_ITERS, schedule=3)
     po, pd = prim.o.cpu().numpy(), prim.d.cpu().numpy()
     ref = ow.cast_simd(po, pd, prim.world_origin, W.MAX_ITERS)
     ref_s = ow.cast(po, pd, prim.world_origin, W.MAX_ITERS, with_pos=True, begin=0, end=min(po.shape[1], 200000))
@@ -93,13 +93,13 @@ def test_device_count_bounds_the_cast(vrt, oracle_mod, cuda):
     scene, cam = W.C1_WORLD.build(), W.c1_camera()
     world = V.World.from_scene(scene)
     prim = cam.rays_device()
-    for sched in (1, 2):
+    for sched in (1, 2, 3):
         for k in (0, 1, 777, prim.n):
             cnt = torch.tensor([k], dtype=torch.int32, device="cuda")
             h = V.HitBatch.empty(prim.n)
             h.rec.fill_(-7)
             ctr = V.new_counters()
-            world.cast(prim, W.MAX_ITERS, schedule=sched, count=cnt, out=h, counters=ctr)
+            world.cast(prim, W.MAX_ITERS, schedule=sched, count=cnt, out=h, counters=ctr, pass_iters=8)
             torch.cuda.synchronize()
             rec = h.rec.cpu().numpy()
             assert int(ctr[0].item()) == k

@@ -13,14 +13,20 @@ from test_oracle import dense_world
 pytestmark = pytest.mark.gpu
 
 GOLDEN = Path(__file__).resolve().parent / "golden"
-SCHEDULES = [(1, 0), (2, 64), (2, 256), (2, 1024)]  # (schedule, chunk)
+# (schedule, chunk or lookups per pass): grid; persistent with 64/256/1024-ray chunks;
+# multipass with the default 16 lookups x 5 passes and with 4 lookups x 8 passes
+SCHEDULES = [(1, 0), (2, 64), (2, 256), (2, 1024), (3, 0), (3, 4)]
 
 
 def gpu_cast(world, o, d, W, max_iters=128, schedule=1, chunk=0, with_pos=False, counters=None):
     from voxelrt_amd import RayBatch
 
     rb = RayBatch.from_numpy(o, d, W)
-    h = world.cast(rb, max_iters, with_pos=with_pos, schedule=schedule, chunk=chunk, counters=counters)
+    if schedule == 3:
+        h = world.cast(rb, max_iters, with_pos=with_pos, schedule=3, pass_iters=chunk, passes=8 if chunk else 0,
+                       counters=counters)
+    else:
+        h = world.cast(rb, max_iters, with_pos=with_pos, schedule=schedule, chunk=chunk, counters=counters)
     torch.cuda.synchronize()
     return h.t.cpu().numpy(), h.rec.cpu().numpy(), (h.pos.cpu().numpy() if h.pos is not None else None)
 
@@ -99,7 +105,7 @@ def test_special_rays(vrt, oracle_mod, cuda):
     for v in (vox, full, empty):
         world, ow = pair_from_dense(vrt, oracle_mod, v)
         ref = ow.cast(o, d, (0, 0, 0), 128)
-        for sched, chunk in SCHEDULES[:2]:
+        for sched, chunk in SCHEDULES[:2] + SCHEDULES[4:]:
             t, rec, _ = gpu_cast(world, o, d, (0, 0, 0), 128, sched, chunk)
             assert_hits_equal(t, rec, ref["t"], ref["rec"], f"special sched={sched}")
             assert np.array_equal(t.view(np.uint32), ref["t"].view(np.uint32))

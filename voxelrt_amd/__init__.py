@@ -15,6 +15,7 @@ from .api import (  # noqa: F401
     incoherent_rays_host,
     new_counters,
     bounce,
+    cast_frame,
     Batch,
     BatchPlan,
     autotune,
@@ -26,6 +27,6 @@ from .api import (  # noqa: F401
 
 __all__ = [
     "VrtError", "CapacityError", "lib", "LIB_PATH", "Scene", "World", "RayBatch", "HitBatch", "HitResult",
-    "CameraSpec", "incoherent_rays_host", "new_counters", "bounce", "Batch", "BatchPlan", "autotune", "render",
+    "CameraSpec", "incoherent_rays_host", "new_counters", "bounce", "cast_frame", "Batch", "BatchPlan", "autotune", "render",
     "render_saved", "render_backward", "sun_direction",
 ]

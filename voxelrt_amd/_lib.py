@@ -20,6 +20,7 @@ VRT_ERR_STATE = 5
 VRT_SCHED_AUTO = 0
 VRT_SCHED_GRID = 1
 VRT_SCHED_PERSISTENT = 2
+VRT_SCHED_MULTIPASS = 3
 
 VRT_SEG_CAMERA = 0
 VRT_SEG_RAYS = 1
@@ -82,7 +83,8 @@ class CastCounters(C.Structure):
 
 class CastOpts(C.Structure):
     _fields_ = [("schedule", u32), ("refill", u32), ("chunk", u32), ("waves_per_cu", u32),
-                ("counters", P), ("count", P)]
+                ("counters", P), ("count", P), ("pass_iters", u32), ("passes", u32), ("scratch", P),
+                ("scratch_bytes", C.c_size_t)]
 
 
 class HitF64(C.Structure):
@@ -91,6 +93,13 @@ class HitF64(C.Structure):
 
 class RayBuffers(C.Structure):
     _fields_ = [("ox", P), ("oy", P), ("oz", P), ("dx", P), ("dy", P), ("dz", P)]
+
+
+class FrameOpts(C.Structure):
+    _fields_ = [("schedule", u32), ("refill", u32), ("chunk", u32), ("counters", P)]
+
+
+VRT_HIT_NO_RAY = 1 << 14
 
 
 class Camera(C.Structure):
@@ -135,11 +144,13 @@ _SIGNATURES = {
     "vrt_world_get_buffers": (C.c_int, [P, C.POINTER(WorldBuffers)]),
     "vrt_cast": (C.c_int, [P, C.POINTER(Rays), u32, C.POINTER(Hits), P]),
     "vrt_cast_ex": (C.c_int, [P, C.POINTER(Rays), u32, C.POINTER(Hits), C.POINTER(CastOpts), P]),
-    "vrt_cast_scratch_bytes": (C.c_size_t, []),
+    "vrt_cast_scratch_bytes": (C.c_size_t, [u32]),
     "vrt_cast_one_f64": (C.c_int, [P, C.POINTER(C.c_double * 3), C.POINTER(C.c_double * 3), u32, C.POINTER(HitF64)]),
     "vrt_bounce_scratch_bytes": (C.c_size_t, [u32]),
     "vrt_bounce_generate": (C.c_int, [C.POINTER(Rays), C.POINTER(Hits), u32, u32, C.POINTER(RayBuffers), P, P, P, P]),
     "vrt_camera_rays": (C.c_int, [C.POINTER(Camera), C.POINTER(RayBuffers), P]),
+    "vrt_cast_frame": (C.c_int, [P, C.POINTER(Camera), u32, u32, C.POINTER(Hits), C.POINTER(Hits),
+                                 C.POINTER(FrameOpts), P]),
     "vrt_batch_plan_create": (C.c_int, [P, C.POINTER(Batch), C.POINTER(BatchTuning), P, C.POINTER(P)]),
     "vrt_batch_launch": (C.c_int, [P, P, P]),
     "vrt_batch_graph_launch": (C.c_int, [P, P]),

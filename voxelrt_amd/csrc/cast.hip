@@ -102,34 +102,6 @@ __global__ __launch_bounds__(256) void cast_grid_kernel(WorldView w, RaysIn r, H
 // statically; further chunks come from its XCD's queue counter and, once that
 // part is drained, from the other parts in turn.  The counters live on separate
 // 256-byte lines, so claims from different XCDs do not serialize.
-static const uint32_t kQueues = 8;
-static const uint32_t kQueueStride = 64;  // u32 per counter (256 bytes)
-
-struct ChunkQueue {
-    uint32_t* ctr;          // kQueues counters, kQueueStride apart, zero at launch
-    uint32_t nchunks;       // total chunks
-    uint32_t waves_per_q;   // waves with a static first chunk in each part
-};
-
-__device__ __forceinline__ uint32_t part_begin(uint32_t q, uint32_t nchunks) {
-    return (uint32_t)(((uint64_t)nchunks * q) / kQueues);
-}
-
-// Next chunk for this wave, or ~0u when every part is drained.  Wave-uniform.
-__device__ __forceinline__ uint32_t claim_chunk(const ChunkQueue& cq, uint32_t home, uint32_t& probe) {
-    while (probe < kQueues) {
-        uint32_t q = (home + probe) % kQueues;
-        uint32_t b = part_begin(q, cq.nchunks), e = part_begin(q + 1, cq.nchunks);
-        uint32_t k = 0;
-        if (__lane_id() == 0) k = atomicAdd(cq.ctr + q * kQueueStride, 1u);
-        k = __shfl(k, 0, 64);
-        uint32_t c = b + cq.waves_per_q + k;
-        if (c < e) return c;
-        probe++;
-    }
-    return ~0u;
-}
-
 // Each wave keeps 64 ray slots.  A finished ray parks its result in registers;
 // once at least refill_min lanes are free (or all of them), the wave writes
 // the parked results and loads new rays into the free lanes from its pool of
@@ -226,6 +198,138 @@ __global__ __launch_bounds__(256) void cast_persistent_kernel(WorldView w, RaysI
     if constexpr (COUNT) flush_counters(ctr, c_rays, c_iters, c_desc, c_hits);
 }
 
+// ---------------------------------------------------------------- MULTIPASS schedule
+
+// A wave runs until its slowest ray, so rays with long-tailed iteration counts
+// (diffuse bounces: median 21, p99 77 lookups on the 1024^3 terrain) keep most
+// lanes idle.  The multi-pass schedule traces every ray for at most `pass_iters`
+// further lookups per pass; rays still running are parked as 20-byte
+// continuation records (ray index, lookups/descents, sideDist) and the next pass
+// packs them densely into waves again.  The last pass runs to the cap.
+// Continuations go to 8 regions (one per XCD: workgroup b % 8) through one
+// atomic per workgroup, so claims never serialize on a single counter.
+static const uint32_t kRegions = 8;
+
+struct ContSet {
+    uint32_t* idx;        // ray index
+    uint32_t* it;         // lookups | descents << 16
+    float* sx; float* sy; float* sz;  // sideDist
+    uint32_t* cnt;        // kRegions counters, kQueueStride apart
+    uint32_t region_cap;  // entries per region
+};
+
+__device__ __forceinline__ uint32_t cont_total(const ContSet& c) {
+    uint32_t t = 0;
+#pragma unroll
+    for (uint32_t r = 0; r < kRegions; r++) t += min(c.cnt[r * kQueueStride], c.region_cap);
+    return t;
+}
+
+template <bool FIRST, bool COUNT>
+__global__ __launch_bounds__(256) void cast_pass_kernel(WorldView w, RaysIn r, HitsOut h, uint32_t cap,
+                                                        uint32_t max_iters, ContSet in, ContSet out,
+                                                        vrt_cast_counters* ctr) {
+    __shared__ uint32_t s_wave[4];
+    __shared__ uint32_t s_base;
+    uint32_t counts[kRegions];
+    uint32_t total;
+    if (FIRST) {
+        total = ray_count(r);
+    } else {
+        total = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < kRegions; k++) {
+            counts[k] = min(in.cnt[k * kQueueStride], in.region_cap);
+            total += counts[k];
+        }
+    }
+    const uint32_t nchunks = (total + 255) / 256;
+    const uint32_t region = blockIdx.x % kRegions;
+    const uint32_t wave = threadIdx.x / 64;
+    unsigned long long c_rays = 0, c_iters = 0, c_desc = 0, c_hits = 0;
+    for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x) {  // uniform per workgroup
+        uint32_t i = c * 256 + threadIdx.x;
+        bool valid = i < total;
+        RayState st;
+        uint32_t idx = 0;
+        if (valid) {
+            if (FIRST) {
+                idx = i;
+                ray_setup(st, r.ox[idx], r.oy[idx], r.oz[idx], r.dx[idx], r.dy[idx], r.dz[idx], r.W[0], r.W[1], r.W[2]);
+            } else {
+                uint32_t e = i, k = 0;
+                while (k + 1 < kRegions && e >= counts[k]) {
+                    e -= counts[k];
+                    k++;
+                }
+                e += k * in.region_cap;
+                idx = in.idx[e];
+                ray_setup(st, r.ox[idx], r.oy[idx], r.oz[idx], r.dx[idx], r.dy[idx], r.dz[idx], r.W[0], r.W[1], r.W[2]);
+                uint32_t packed = in.it[e];
+                st.iters = packed & 0xFFFFu;
+                st.descents = packed >> 16;
+                st.side[0] = in.sx[e];
+                st.side[1] = in.sy[e];
+                st.side[2] = in.sz[e];
+                // the position after the parked step, recomputed exactly as the step computed it
+                float tmin = min_x86(min_x86(st.side[0], st.side[1]), st.side[2]) + 0.001f;
+#pragma unroll
+                for (int a = 0; a < 3; a++) st.cur[a] = fmaf(tmin, st.d[a], st.o[a]);
+            }
+        }
+        bool wave_tame = __all(!valid || ray_is_tame(st));
+        int res = STEP_CONTINUE;
+        int32_t v[3] = {0, 0, 0};
+        uint32_t slot = 0;
+        if (valid) {
+            while (st.iters < cap) {
+                res = wave_tame ? ray_step<false>(w, st, v, &slot) : ray_step<true>(w, st, v, &slot);
+                if (res != STEP_CONTINUE) break;
+            }
+        }
+        bool survivor = valid && res == STEP_CONTINUE && st.iters < max_iters;
+        if (valid && !survivor) {
+            float t;
+            int4 rec;
+            if (res == STEP_HIT)
+                ray_hit_record(w, st, v, slot, t, rec);
+            else
+                ray_miss_record(st, t, rec);
+            store_result(h, idx, st, t, rec);
+            if constexpr (COUNT) {
+                c_rays++;
+                c_iters += st.iters;
+                c_desc += st.descents;
+                c_hits += (rec.w & 0xFF) != 0;
+            }
+        }
+        // append survivors: one atomic per workgroup into this workgroup's region
+        uint64_t sb = __ballot(survivor);
+        if (__lane_id() == 0) s_wave[wave] = (uint32_t)__popcll(sb);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t tot = s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
+            s_base = tot ? atomicAdd(out.cnt + region * kQueueStride, tot) : 0u;
+        }
+        __syncthreads();
+        if (survivor) {
+            uint32_t off = s_base;
+            for (uint32_t k = 0; k < wave; k++) off += s_wave[k];
+            off += __builtin_amdgcn_mbcnt_hi((uint32_t)(sb >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)sb, 0u));
+            if (off < out.region_cap) {  // always true: a region holds every ray its workgroups can see
+                uint32_t e = region * out.region_cap + off;
+                out.idx[e] = idx;
+                out.it[e] = (st.iters & 0xFFFFu) | (st.descents << 16);
+                out.sx[e] = st.side[0];
+                out.sy[e] = st.side[1];
+                out.sz[e] = st.side[2];
+            }
+        }
+        __syncthreads();  // s_wave / s_base are reused by the next chunk
+    }
+    if constexpr (COUNT) flush_counters(ctr, c_rays, c_iters, c_desc, c_hits);
+}
+
 __global__ void fill_u32_kernel(uint32_t* p, uint32_t n, uint32_t v) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -260,6 +364,41 @@ int device_info(int dev, int* cus, uint32_t** queue) {
     return VRT_OK;
 }
 
+// Library-owned continuation scratch, one per device, grown on demand.
+struct PassScratch {
+    void* buf = nullptr;
+    size_t bytes = 0;
+};
+static PassScratch g_pass[64];
+static std::mutex g_pass_mu;
+
+// Continuation entries a region must hold for a cast of n rays over G workgroups.
+static uint32_t region_capacity(uint32_t n, uint32_t grid) {
+    uint32_t nchunks = (n + 255) / 256;
+    (void)grid;
+    return ((nchunks + kRegions - 1) / kRegions) * 256;
+}
+
+size_t pass_scratch_bytes(uint32_t n) {
+    size_t per_set = (size_t)region_capacity(n, 0) * kRegions * 20 + kRegions * kQueueStride * 4;
+    return 2 * per_set + 256;
+}
+
+static void carve(void* base, uint32_t region_cap, ContSet& a, ContSet& b) {
+    size_t ents = (size_t)region_cap * kRegions;
+    char* p = (char*)(((uintptr_t)base + 255) & ~(uintptr_t)255);
+    ContSet* sets[2] = {&a, &b};
+    for (ContSet* c : sets) {
+        c->cnt = (uint32_t*)p; p += kRegions * kQueueStride * 4;
+        c->idx = (uint32_t*)p; p += ents * 4;
+        c->it = (uint32_t*)p; p += ents * 4;
+        c->sx = (float*)p; p += ents * 4;
+        c->sy = (float*)p; p += ents * 4;
+        c->sz = (float*)p; p += ents * 4;
+        c->region_cap = region_cap;
+    }
+}
+
 // ---------------------------------------------------------------- launch
 
 int launch_cast(const vrt_world* world, const RaysIn& r, const HitsOut& h, uint32_t max_iters,
@@ -274,6 +413,56 @@ int launch_cast(const vrt_world* world, const RaysIn& r, const HitsOut& h, uint3
         else
             cast_grid_kernel<false><<<blocks, 256, 0, s>>>(w, r, h, max_iters, nullptr);
         VRT_HIP(hipGetLastError());
+        return VRT_OK;
+    }
+    if (sched == VRT_SCHED_MULTIPASS) {
+        uint32_t K = o.pass_iters ? o.pass_iters : 16;
+        uint32_t P = o.passes ? o.passes : 5;
+        if (P > 64) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_cast_ex: at most 64 passes");
+        int cus = 0;
+        int rc = device_info(world->device, &cus, nullptr);
+        if (rc != VRT_OK) return rc;
+        size_t need = pass_scratch_bytes(r.n);
+        void* base = o.scratch;
+        if (base) {
+            if (o.scratch_bytes < need) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_cast_ex: scratch smaller than vrt_cast_scratch_bytes(n)");
+        } else {
+            std::lock_guard<std::mutex> lk(g_pass_mu);
+            PassScratch& ps = g_pass[world->device];
+            if (ps.bytes < need) {
+                VRT_HIP(hipStreamSynchronize(s));
+                (void)hipFree(ps.buf);
+                ps.buf = nullptr;
+                ps.bytes = 0;
+                VRT_HIP(hipMalloc(&ps.buf, need));
+                ps.bytes = need;
+            }
+            base = ps.buf;
+        }
+        ContSet a, b;
+        carve(base, region_capacity(r.n, 0), a, b);
+        uint32_t grid = std::max<uint32_t>(kRegions, std::min<uint32_t>((uint32_t)cus * 8, (r.n + 255) / 256));
+        grid = (grid + kRegions - 1) / kRegions * kRegions;
+        ContSet* cur_in = &a;
+        ContSet* cur_out = &b;
+        for (uint32_t p = 0; p < P; p++) {
+            uint32_t cap = p + 1 == P ? max_iters : std::min<uint64_t>((uint64_t)K * (p + 1), max_iters);
+            fill_u32_kernel<<<(kRegions * kQueueStride + 255) / 256, 256, 0, s>>>(cur_out->cnt, kRegions * kQueueStride, 0u);
+            if (p == 0) {
+                if (o.counters)
+                    cast_pass_kernel<true, true><<<grid, 256, 0, s>>>(w, r, h, cap, max_iters, *cur_in, *cur_out, o.counters);
+                else
+                    cast_pass_kernel<true, false><<<grid, 256, 0, s>>>(w, r, h, cap, max_iters, *cur_in, *cur_out, nullptr);
+            } else {
+                if (o.counters)
+                    cast_pass_kernel<false, true><<<grid, 256, 0, s>>>(w, r, h, cap, max_iters, *cur_in, *cur_out, o.counters);
+                else
+                    cast_pass_kernel<false, false><<<grid, 256, 0, s>>>(w, r, h, cap, max_iters, *cur_in, *cur_out, nullptr);
+            }
+            VRT_HIP(hipGetLastError());
+            std::swap(cur_in, cur_out);
+            if (cap >= max_iters) break;
+        }
         return VRT_OK;
     }
     if (sched != VRT_SCHED_PERSISTENT) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_cast_ex: unknown schedule");
@@ -432,7 +621,7 @@ int vrt_cast_ex(const vrt_world* world, const vrt_rays* rays, uint32_t max_iters
     });
 }
 
-size_t vrt_cast_scratch_bytes(void) { return kQueueRing * sizeof(uint32_t); }
+size_t vrt_cast_scratch_bytes(uint32_t n) { return pass_scratch_bytes(n); }
 
 int vrt_cast_one_f64(const vrt_world* world, const double origin[3], const double dir[3], uint32_t max_iters,
                      vrt_hit_f64* out) {

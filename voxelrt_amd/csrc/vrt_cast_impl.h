@@ -33,6 +33,36 @@ struct HitsOut {
     float* __restrict__ pos_z;
 };
 
+// ---------------------------------------------------------------- work queues
+constexpr uint32_t kQueues = 8;
+constexpr uint32_t kQueueStride = 64;  // u32 per counter (256 bytes)
+
+struct ChunkQueue {
+    uint32_t* ctr;          // kQueues counters, kQueueStride apart, zero at launch
+    uint32_t nchunks;       // total chunks
+    uint32_t waves_per_q;   // waves with a static first chunk in each part
+};
+
+__device__ __forceinline__ uint32_t part_begin(uint32_t q, uint32_t nchunks) {
+    return (uint32_t)(((uint64_t)nchunks * q) / kQueues);
+}
+
+// Next chunk for this wave, or ~0u when every part is drained.  Wave-uniform.
+__device__ __forceinline__ uint32_t claim_chunk(const ChunkQueue& cq, uint32_t home, uint32_t& probe) {
+    while (probe < kQueues) {
+        uint32_t q = (home + probe) % kQueues;
+        uint32_t b = part_begin(q, cq.nchunks), e = part_begin(q + 1, cq.nchunks);
+        uint32_t k = 0;
+        if (__lane_id() == 0) k = atomicAdd(cq.ctr + q * kQueueStride, 1u);
+        k = __shfl(k, 0, 64);
+        uint32_t c = b + cq.waves_per_q + k;
+        if (c < e) return c;
+        probe++;
+    }
+    return ~0u;
+}
+
+
 // Validates the ABI structs and converts them to kernel views.
 int check_rays_hits(const vrt_rays* rays, const vrt_hits* hits, RaysIn& r, HitsOut& h);
 // Launches a cast with the given options on stream s.
