@@ -63,7 +63,7 @@ typedef struct vrt_world_desc {
     uint32_t log2_size_xz;   /* voxel extent along x and z is 1 << log2_size_xz, 5..14 */
     uint32_t log2_size_y;    /* voxel extent along y is 1 << log2_size_y, 5..14 */
     uint32_t initial_slots;  /* device brick slots reserved up front (0 = grow on demand) */
-    uint32_t max_slots;      /* slot arena capacity (0 = 64 per sector, i.e. never exhausted) */
+    uint32_t max_slots;      /* slot arena capacity (0 = 64 per sector, i.e. never exhausted); at most 2^28 */
 } vrt_world_desc;
 
 /* Shape of a created world (all derived from the descriptor). */
@@ -205,6 +205,40 @@ typedef struct vrt_hit_f64 {
 } vrt_hit_f64;
 int vrt_cast_one_f64(const vrt_world* world, const double origin[3], const double dir[3],
                      uint32_t max_iters, vrt_hit_f64* out);
+
+/* ---------------------------------------------------------------- shader method */
+
+/*
+ * The GLSL traversal (src/VoxelRT/Shaders/VoxelTraversal.glsl) on the same world:
+ * a different method from vrt_cast, not bit-compatible with it.  Replaces the
+ * shader's rayCast (:162-204) and rayCastCoarse (:206-243):
+ *   - the ray starts at its origin clipped to the world box inset by one voxel
+ *     (clipRayToAABB, :133-145) and first steps out of that voxel;
+ *   - getStepPos (:92-131) adds a sector-of-sectors level: a sector whose brick
+ *     mask is 0 steps 32/64/128 voxels from the 4x4x4-sector group mask;
+ *   - ray_cell_lut = 1 first ANDs each mask with the ray-cell interaction table
+ *     of the ray's octant (u_UseAnisotropicLods, :116-121; GpuRenderer.cpp:192-210);
+ *   - full: +5 ulp bias on tmin, cap 256; coarse: +0.001 bias, cap 96, and from
+ *     the 31st iteration a non-empty 4x4x4 voxel cell counts as a hit at its first
+ *     solid voxel (findAnyOccupiedPos, :40-52);
+ *   - t is the biased tmin; normal = -sign(d) on every axis with tmin >= side;
+ *     iterations = the loop index at the hit or miss (the cap when exhausted).
+ * FP32 with IEEE division, a*b+c as fmaf, min/max as fminf/fmaxf.
+ */
+typedef enum vrt_shader_method {
+    VRT_SHADER_FULL = 0,    /* rayCast */
+    VRT_SHADER_COARSE = 1   /* rayCastCoarse (shadow and bounce rays) */
+} vrt_shader_method;
+
+typedef struct vrt_shader_opts {
+    uint32_t method;        /* vrt_shader_method */
+    uint32_t ray_cell_lut;  /* 1 = mask with the ray-cell interaction table */
+    uint32_t max_iters;     /* 0 = the shader's cap (256 full, 96 coarse), else 1..65535 */
+    uint32_t reserved;      /* 0 */
+} vrt_shader_opts;
+
+int vrt_cast_shader(const vrt_world* world, const vrt_rays* rays, const vrt_shader_opts* opts,
+                    vrt_hits* hits, void* stream);
 
 /* ---------------------------------------------------------------- bounce rays */
 

@@ -39,6 +39,10 @@ typedef struct oracle_world {
     uint8_t* storage;             /* [nsec * 32768] */
     uint64_t* occupancy;          /* [nsec * 512] */
     uint64_t palette[256];
+    /* sector-of-sectors masks (GpuRenderer.cpp:133-149): one u64 per 4x4x4 group of
+     * sectors, bit = that sector has a brick; group grid of max(1, n / 4) per axis */
+    uint32_t gshift_xz, gshift_y;
+    uint64_t* sector_groups;      /* [1 << (2 * gshift_xz + gshift_y)] */
 } oracle_world;
 
 /* Hit record in the same packing as the product's vrt_hit (include/vrt.h). */
@@ -98,6 +102,30 @@ void oracle_cast_f64(const oracle_world* w, const double origin[3], const double
 int oracle_dda_f64(const oracle_world* w, const double o[3], const double d[3],
                    const int32_t world_origin[3], uint32_t max_steps,
                    int32_t voxel[3], double* t, double* min_gap);
+
+/* ---- shader method: the GLSL traversal (src/VoxelRT/Shaders/VoxelTraversal.glsl) ----
+ *
+ * Contract (FP32, IEEE division; GLSL's a*b+c taken as contracted):
+ *   inv = 1/d; tStart = (step(0,d) - o) * inv;
+ *   clipRayToAABB (:133-145) against [1 - W, size - W - 1], per-axis min/max = fminf/fmaxf,
+ *     start = fmaf(d, tmin, o) when 0 < tmin < tmax, else o;
+ *   rayCast (:162-204): v = W + floor(start); each iteration side = fmaf(v - W, inv, tStart),
+ *     tmin = bits(fminf(fminf(side))) + 5 ulp, cur = fmaf(tmin, d, o), v = W + floor(cur),
+ *     bounds, getStepPos; cap 256; distance = biased tmin; iterations = loop index;
+ *   rayCastCoarse (:206-243): o clipped first, tStart from the clipped origin, +0.001 bias,
+ *     cap 96, coarse hits (findAnyOccupiedPos, :40-52) from loop index 31 on;
+ *   getStepPos (:92-131): brick / voxel levels as the CPU path, plus the sector-of-sectors
+ *     level (scale 32: LOD 32/64/128) when the sector mask is 0; with the ray-cell
+ *     interaction LUT (:116-121) the mask is first ANDed with the octant's entry;
+ *   normal = tmin >= side ? -sign(d) : 0 per axis (:194-196), in the code of vrt_hit;
+ *   floor to int as floor2i.  A miss after the cap reports iterations = cap.
+ * method: 0 = rayCast, 1 = rayCastCoarse.  max_iters 0 = the shader's cap. */
+void oracle_ray_cell_lut(uint64_t table[512]);
+void oracle_cast_shader(const oracle_world* w, const float* const o[3], const float* const d[3],
+                        const int32_t world_origin[3], uint32_t begin, uint32_t end, uint32_t method,
+                        uint32_t lut, uint32_t max_iters, float* t, oracle_hit* rec, float* const pos[3]);
+/* Internal: fills w->sector_groups from w->sector_masks. */
+int oracle_world_build_groups(oracle_world* w);
 
 /* PCG of RandomGen.glsl:6-20. */
 uint32_t oracle_pcg_init(uint32_t px, uint32_t py, uint32_t seed);

@@ -66,6 +66,9 @@ def lib() -> C.CDLL:
         h.oracle_render_backward.argtypes = [u32, P, P, P, P, P]
         h.oracle_render_loss.argtypes = [u32, P, P, P, P]
         h.oracle_render_loss.restype = C.c_double
+        h.oracle_ray_cell_lut.argtypes = [P]
+        h.oracle_cast_shader.argtypes = [P, C.POINTER(P * 3), C.POINTER(P * 3), C.POINTER(C.c_int32 * 3), u32, u32,
+                                         u32, u32, u32, P, P, C.c_void_p]
         _lib = h
     return _lib
 
@@ -136,6 +139,23 @@ class OracleWorld:
             raise RuntimeError("host has neither AVX-512 nor AVX2+FMA")
         return {"t": t, "rec": rec, "lanes": lanes}
 
+    def cast_shader(self, o, d, world_origin=(0, 0, 0), method="full", lut=False, max_iters=0, begin=0, end=None,
+                    with_pos=False) -> dict:
+        """Shader-method cast (VoxelTraversal.glsl rayCast / rayCastCoarse) of rays [begin, end)."""
+        o = np.ascontiguousarray(o, dtype=np.float32)
+        d = np.ascontiguousarray(d, dtype=np.float32)
+        n = o.shape[1]
+        end = n if end is None else end
+        t = np.zeros(n, dtype=np.float32)
+        rec = np.zeros((n, 4), dtype=np.int32)
+        pos = np.zeros((3, n), dtype=np.float32) if with_pos else None
+        W = (C.c_int32 * 3)(*world_origin)
+        m = {"full": 0, "coarse": 1}[method]
+        lib().oracle_cast_shader(self._h, C.byref(_p3(o)), C.byref(_p3(d)), C.byref(W), begin, end, m, int(lut),
+                                 max_iters, t.ctypes.data, rec.ctypes.data,
+                                 C.cast(C.byref(_p3(pos)), C.c_void_p) if with_pos else None)
+        return {"t": t[begin:end], "rec": rec[begin:end], "pos": pos[:, begin:end] if with_pos else None}
+
     def cast_f64(self, origin, direction, max_iters=1024) -> dict:
         o = (C.c_double * 3)(*origin)
         d = (C.c_double * 3)(*direction)
@@ -164,6 +184,13 @@ class OracleWorld:
             self.close()
         except Exception:
             pass
+
+
+def ray_cell_lut() -> np.ndarray:
+    """The ray-cell interaction LUT (GpuRenderer.cpp:192-210), [8 octants * 64 cells] u64."""
+    t = np.zeros(512, dtype=np.uint64)
+    lib().oracle_ray_cell_lut(t.ctypes.data)
+    return t
 
 
 def simd_width() -> int:

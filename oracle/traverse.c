@@ -86,6 +86,10 @@ int oracle_world_create(uint32_t log2_size_xz, uint32_t log2_size_y, uint32_t ns
             src += 512;
         }
     }
+    if (oracle_world_build_groups(w)) {
+        oracle_world_destroy(w);
+        return 2;
+    }
     *out = w;
     return 0;
 }
@@ -95,6 +99,7 @@ void oracle_world_destroy(oracle_world* w) {
     free(w->sector_masks);
     free(w->storage);
     free(w->occupancy);
+    free(w->sector_groups);
     free(w);
 }
 

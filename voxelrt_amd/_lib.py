@@ -95,6 +95,14 @@ class RayBuffers(C.Structure):
     _fields_ = [("ox", P), ("oy", P), ("oz", P), ("dx", P), ("dy", P), ("dz", P)]
 
 
+VRT_SHADER_FULL = 0
+VRT_SHADER_COARSE = 1
+
+
+class ShaderOpts(C.Structure):
+    _fields_ = [("method", u32), ("ray_cell_lut", u32), ("max_iters", u32), ("reserved", u32)]
+
+
 class FrameOpts(C.Structure):
     _fields_ = [("schedule", u32), ("refill", u32), ("chunk", u32), ("counters", P)]
 
@@ -149,6 +157,7 @@ _SIGNATURES = {
     "vrt_bounce_scratch_bytes": (C.c_size_t, [u32]),
     "vrt_bounce_generate": (C.c_int, [C.POINTER(Rays), C.POINTER(Hits), u32, u32, C.POINTER(RayBuffers), P, P, P, P]),
     "vrt_camera_rays": (C.c_int, [C.POINTER(Camera), C.POINTER(RayBuffers), P]),
+    "vrt_cast_shader": (C.c_int, [P, C.POINTER(Rays), C.POINTER(ShaderOpts), C.POINTER(Hits), P]),
     "vrt_cast_frame": (C.c_int, [P, C.POINTER(Camera), u32, u32, C.POINTER(Hits), C.POINTER(Hits),
                                  C.POINTER(FrameOpts), P]),
     "vrt_batch_plan_create": (C.c_int, [P, C.POINTER(Batch), C.POINTER(BatchTuning), P, C.POINTER(P)]),

@@ -293,6 +293,19 @@ class World:
                                     _stream_handle(stream)))
         return hits
 
+    def cast_shader(self, rays: RayBatch, method: str = "full", ray_cell_lut: bool = False, max_iters: int = 0,
+                    with_pos: bool = False, out: HitBatch | None = None, stream=None) -> HitBatch:
+        """The GLSL traversal (VoxelTraversal.glsl rayCast / rayCastCoarse): method "full" or "coarse";
+        ray_cell_lut masks with the ray-cell interaction table (u_UseAnisotropicLods); max_iters 0 is
+        the shader's cap (256 / 96)."""
+        m = {"full": L.VRT_SHADER_FULL, "coarse": L.VRT_SHADER_COARSE}[method]
+        hits = out if out is not None else HitBatch.empty(rays.n, with_pos, rays.o.device)
+        r = rays.to_c()
+        h = hits.to_c()
+        opts = L.ShaderOpts(m, int(ray_cell_lut), max_iters, 0)
+        L.check(L.lib().vrt_cast_shader(self._h, C.byref(r), C.byref(opts), C.byref(h), _stream_handle(stream)))
+        return hits
+
     def ray_cast(self, origin, direction, max_iters: int = 1024) -> HitResult:
         """HitResult VoxelMap::RayCast(dvec3 origin, dvec3 dir, uint32_t maxIters = 1024) (VoxelMap.h:214)."""
         o = (C.c_double * 3)(*origin)

@@ -115,27 +115,29 @@ __device__ __forceinline__ void trace_block(const WorldView& w, const BatchArgs&
         tame = tame && (done[k] || ray_is_tame(st[k]));
     }
     bool wave_tame = __all(tame);
-    int res[RPL];
+    bool hit[RPL];
     int32_t v[RPL][3];
-    uint32_t slot[RPL];
 #pragma unroll
-    for (int k = 0; k < RPL; k++) res[k] = STEP_MISS;
-    for (;;) {
-        bool any_running = false;
+    for (int k = 0; k < RPL; k++) hit[k] = false;
+    if constexpr (RPL == 1) {
+        if (!done[0])
+            hit[0] = wave_tame ? ray_trace<false>(w, st[0], a.max_iters, v[0]) : ray_trace<true>(w, st[0], a.max_iters, v[0]);
+    } else {
+        // the lane's rays step in lockstep, so one counter caps them all
+        uint32_t unused;
+        for (uint32_t it = 0; it < a.max_iters; it++) {
+            bool any_running = false;
 #pragma unroll
-        for (int k = 0; k < RPL; k++) {
-            if (!done[k]) {
-                if (st[k].iters >= a.max_iters)
-                    res[k] = STEP_MISS;
-                else if (wave_tame)
-                    res[k] = ray_step<false>(w, st[k], v[k], &slot[k]);
-                else
-                    res[k] = ray_step<true>(w, st[k], v[k], &slot[k]);
-                done[k] = res[k] != STEP_CONTINUE;
+            for (int k = 0; k < RPL; k++) {
+                if (!done[k]) {
+                    bool cont = wave_tame ? ray_advance<false>(w, st[k], v[k], hit[k], &unused)
+                                          : ray_advance<true>(w, st[k], v[k], hit[k], &unused);
+                    done[k] = !cont;
+                }
+                any_running = any_running || !done[k];
             }
-            any_running = any_running || !done[k];
+            if (!any_running) break;
         }
-        if (!any_running) break;
     }
     unsigned long long c_rays = 0, c_iters = 0, c_desc = 0, c_hits = 0;
 #pragma unroll
@@ -143,8 +145,8 @@ __device__ __forceinline__ void trace_block(const WorldView& w, const BatchArgs&
         if (lr[k].valid) {
             float t;
             int4 rec;
-            if (res[k] == STEP_HIT)
-                ray_hit_record(w, st[k], v[k], slot[k], t, rec);
+            if (hit[k])
+                ray_hit_record(w, st[k], v[k], brick_slot_at(w, v[k]), t, rec);
             else
                 ray_miss_record(st[k], t, rec);
             a.h.t[lr[k].out] = t;

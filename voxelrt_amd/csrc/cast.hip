@@ -57,14 +57,8 @@ __device__ __forceinline__ void store_result(const HitsOut& h, uint32_t i, const
 template <bool EXACT>
 __device__ __forceinline__ void trace_full(const WorldView& w, RayState& st, uint32_t max_iters, float& t, int4& rec) {
     int32_t v[3];
-    uint32_t slot = 0;
-    int res = STEP_MISS;
-    for (uint32_t it = 0; it < max_iters; it++) {
-        res = ray_step<EXACT>(w, st, v, &slot);
-        if (res != STEP_CONTINUE) break;
-    }
-    if (res == STEP_HIT)
-        ray_hit_record(w, st, v, slot, t, rec);
+    if (ray_trace<EXACT>(w, st, max_iters, v))
+        ray_hit_record(w, st, v, brick_slot_at(w, v), t, rec);
     else
         ray_miss_record(st, t, rec);
 }

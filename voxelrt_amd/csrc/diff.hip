@@ -25,9 +25,7 @@ __global__ __launch_bounds__(256) void render_fwd_kernel(WorldView w, vrt_camera
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     bool valid = i < n;
     RayState st;
-    int res = STEP_MISS;
     int32_t v[3] = {0, 0, 0};
-    uint32_t slot = 0;
     bool tame = true;
     if (valid) {
         uint32_t px, py;
@@ -40,15 +38,13 @@ __global__ __launch_bounds__(256) void render_fwd_kernel(WorldView w, vrt_camera
     }
     bool wave_tame = __all(tame);
     if (valid) {
-        for (uint32_t it = 0; it < max_iters; it++) {
-            res = wave_tame ? ray_step<false>(w, st, v, &slot) : ray_step<true>(w, st, v, &slot);
-            if (res != STEP_CONTINUE) break;
-        }
+        bool hit = wave_tame ? ray_trace<false>(w, st, max_iters, v) : ray_trace<true>(w, st, max_iters, v);
         float wgt = 0.0f;
         uint32_t gv = 0xFFFFFFFFu;
-        if (res == STEP_HIT) {
+        if (hit) {
             float t;
             int4 rec;
+            uint32_t slot = brick_slot_at(w, v);
             ray_hit_record(w, st, v, slot, t, rec);
             float nx = normal_component((uint32_t)rec.w, 0), ny = normal_component((uint32_t)rec.w, 1),
                   nz = normal_component((uint32_t)rec.w, 2);

@@ -93,18 +93,21 @@ __device__ __forceinline__ void flush4(vrt_cast_counters* ctr, unsigned long lon
 
 // ---------------------------------------------------------------- GRID
 
-template <bool COUNT>
-__device__ __forceinline__ int trace_wave(const WorldView& w, RayState& st, bool active, uint32_t max_iters,
-                                          int32_t v[3], uint32_t& slot) {
+// Traces the wave's active rays, with the fast step when all of them are tame;
+// true on a hit, with v the hit voxel.
+__device__ __forceinline__ bool trace_wave(const WorldView& w, RayState& st, bool active, uint32_t max_iters,
+                                           int32_t v[3]) {
     bool tame = __all(!active || ray_is_tame(st));
-    int res = STEP_MISS;
-    if (active) {
-        for (uint32_t it = 0; it < max_iters; it++) {
-            res = tame ? ray_step<false>(w, st, v, &slot) : ray_step<true>(w, st, v, &slot);
-            if (res != STEP_CONTINUE) break;
-        }
-    }
-    return res;
+    if (!active) return false;
+    return tame ? ray_trace<false>(w, st, max_iters, v) : ray_trace<true>(w, st, max_iters, v);
+}
+
+__device__ __forceinline__ void finish_trace(const WorldView& w, const RayState& st, bool hit, const int32_t v[3],
+                                             float& t, int4& rec) {
+    if (hit)
+        ray_hit_record(w, st, v, brick_slot_at(w, v), t, rec);
+    else
+        ray_miss_record(st, t, rec);
 }
 
 template <bool COUNT>
@@ -114,27 +117,26 @@ __global__ __launch_bounds__(256) void frame_grid_kernel(WorldView w, FrameArgs 
     unsigned long long c[4] = {0, 0, 0, 0};
     RayState st;
     int32_t v[3] = {0, 0, 0};
-    uint32_t slot = 0;
     if (valid) setup_primary(a, i, st);
-    int res = trace_wave<COUNT>(w, st, valid, a.max_iters, v, slot);
+    bool hit = trace_wave(w, st, valid, a.max_iters, v);
     bool bounce = false;
     uint32_t info = 0;
     if (valid) {
         float t;
         int4 rec;
-        finish_record(w, st, res, v, slot, t, rec);
+        finish_trace(w, st, hit, v, t, rec);
         store_hit(a.hp, i, st.cur, t, rec);
         count_ray<COUNT>(c, st, rec);
         info = (uint32_t)rec.w;
         bounce = (info & 0xFF) != 0;
         if (bounce) setup_bounce(a, i, info, st);
     }
-    res = trace_wave<COUNT>(w, st, bounce, a.max_iters, v, slot);
+    hit = trace_wave(w, st, bounce, a.max_iters, v);
     if (valid) {
         if (bounce) {
             float t;
             int4 rec;
-            finish_record(w, st, res, v, slot, t, rec);
+            finish_trace(w, st, hit, v, t, rec);
             store_hit(a.hb, i, st.cur, t, rec);
             count_ray<COUNT>(c, st, rec);
         } else {

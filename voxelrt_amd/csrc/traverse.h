@@ -50,6 +50,8 @@ __device__ __forceinline__ void ray_setup(RayState& r, float ox, float oy, float
         r.side[a] = 0.0f;
         r.cur[a] = r.o[a];
         r.dm[a] = r.d[a] < 0.0f ? 0 : -1;
+        // a per-lane word, so the cell alignment below is one bitfield insert per axis
+        asm volatile("" : "+v"(r.dm[a]));
     }
     r.iters = 0;
     r.descents = 0;
@@ -66,17 +68,46 @@ __device__ __forceinline__ bool ray_is_tame(const RayState& r) {
     return ok && len2 >= 0.25f && len2 <= 4.0f;
 }
 
+// (mask & a) | (~mask & b): V_BFI_B32.
+__device__ __forceinline__ uint32_t bitfield_insert(uint32_t mask, uint32_t a, uint32_t b) {
+    return (mask & a) | (~mask & b);
+}
+
+// floor(x) as int32 in one V_CVT_FLR_I32_F32; x finite and |x| < 2^31.
+__device__ __forceinline__ int32_t floor_i32_in_range(float x) {
+    int32_t f;
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(f) : "v"(x));
+    return f;
+}
+
 // One iteration of RayCast (CpuRenderer.cpp:185-202) with GetStepPos (:135-171).
 // On STEP_HIT, v is the hit voxel and *hit_slot its brick slot.
+//
+// Integer work is arranged for few VALU instructions per iteration:
+//   mask bit test: below = m << (63 - bi) moves bit bi to bit 63 and drops the
+//     bits above it, so the test is a sign test and popcount(below) counts the
+//     bricks at or below bi, i.e. slot + 1 (the occupancy load subtracts the
+//     one as a constant offset);
+//   index fields: only bits 0..5 of the in-mask indices are meaningful; the
+//     64-bit shifts read only those bits (the & 63 costs nothing), and the
+//     fields are merged with bitfield inserts;
+//   cell mask: cm = 2^s - 1 with s = level + (group empty) + (mask empty), the
+//     reference's LOD of +1 / +2 (a 4x4x4 mask that is 0 has every group empty);
+//   alignment: (v & ~cm) | (dm & cm), one bitfield insert per axis.
+// Occupancy indices slot * 8 + cell are 32-bit: worlds hold at most 2^28 slots.
 template <bool EXACT>
-__device__ __forceinline__ int ray_step(const WorldView& w, RayState& r, int32_t v[3], uint32_t* hit_slot) {
+__device__ __forceinline__ bool ray_advance(const WorldView& w, RayState& r, int32_t v[3], bool& hit,
+                                            uint32_t* hit_slot) {
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-        int32_t f = EXACT ? floor2i(r.cur[a]) : (int32_t)floorf(r.cur[a]);
+        int32_t f = EXACT ? floor2i(r.cur[a]) : floor_i32_in_range(r.cur[a]);
         v[a] = (int32_t)((uint32_t)r.W[a] + (uint32_t)f);
     }
     // GetInboundMask (:114-117)
-    if (!((uint32_t)(v[0] | v[2]) < w.size_xz && (uint32_t)v[1] < w.size_y)) return STEP_MISS;
+    if (!((uint32_t)(v[0] | v[2]) < w.size_xz && (uint32_t)v[1] < w.size_y)) {
+        hit = false;
+        return false;
+    }
     r.iters++;
     uint32_t ux = (uint32_t)v[0], uy = (uint32_t)v[1], uz = (uint32_t)v[2];
     uint32_t si = (ux >> 5) | (uz >> 5) << w.shift_xz | (uy >> 5) << (2 * w.shift_xz);
@@ -85,31 +116,34 @@ __device__ __forceinline__ int ray_step(const WorldView& w, RayState& r, int32_t
     // one load (the occupancy mask) instead of two
     asm volatile("" : "+v"(sec.z));
     uint64_t m = (uint64_t)sec.x | (uint64_t)sec.y << 32;
-    uint32_t bi = __builtin_amdgcn_ubfe(ux, 3, 2) | __builtin_amdgcn_ubfe(uz, 3, 2) << 2 |
-                  __builtin_amdgcn_ubfe(uy, 3, 2) << 4;
-    uint32_t unit = 8;  // cell edge at the level below the mask: a brick
-    if ((uint32_t)(m >> bi) & 1u) {
+    // brick within the sector: bits 3..4 of x, z, y at bits 0..1, 2..3, 4..5
+    uint32_t bi = bitfield_insert(15u, bitfield_insert(3u, __builtin_amdgcn_ubfe(ux, 3, 2), uz >> 1), uy << 1);
+    uint32_t level = 3;  // log2 of the cell edge below the mask: a brick, 8 voxels
+    uint64_t below = m << ((63u - bi) & 63u);
+    if ((int32_t)(below >> 32) < 0) {
         r.descents++;
-        uint32_t slot = sec.z + (uint32_t)__popcll(m & ((1ull << bi) - 1ull));
+        uint32_t slot1 = sec.z + (uint32_t)__popcll(below);
         uint32_t cell = __builtin_amdgcn_ubfe(ux, 2, 1) | __builtin_amdgcn_ubfe(uz, 2, 1) << 1 |
                         __builtin_amdgcn_ubfe(uy, 2, 1) << 2;
-        m = w.occ[(size_t)slot * 8 + cell];
-        bi = (ux & 3u) | (uz & 3u) << 2 | (uy & 3u) << 4;
-        unit = 1;
-        if ((uint32_t)(m >> bi) & 1u) {
-            *hit_slot = slot;
-            return STEP_HIT;
+        m = w.occ[(size_t)(slot1 * 8u + cell) - 8u];
+        // voxel within the brick's 4x4x4 cell: bits 0..1 of x, z, y
+        bi = bitfield_insert(15u, bitfield_insert(3u, ux, uz << 2), uy << 4);
+        level = 0;
+        if ((int32_t)((m << ((63u - bi) & 63u)) >> 32) < 0) {
+            *hit_slot = slot1 - 1;
+            hit = true;
+            return false;
         }
     }
-    // level4: the whole 4x4x4 mask is empty; level2: the 2x2x2 group holding bi is empty
-    // ((half >> (bi & 0xA)) & 0x00330033 on the 32-bit half selected by bit 5 of bi)
+    // the 2x2x2 group holding bi: (half >> (bi & 0xA)) & 0x00330033 on the 32-bit
+    // half selected by bit 5 of bi
     uint32_t grp = (uint32_t)(m >> (bi & 0x2Au)) & 0x00330033u;
-    uint32_t k = m == 0 ? 2u : (grp == 0 ? 1u : 0u);
-    int32_t cm = (int32_t)((unit << k) - 1u);
+    uint32_t s = level + (grp == 0 ? 1u : 0u) + (m == 0 ? 1u : 0u);
+    uint32_t cm = (1u << s) - 1u;
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         // d < 0: v & ~cm (near corner); d >= 0: v | cm (far corner)
-        v[a] = (v[a] & ~cm) | (cm & r.dm[a]);
+        v[a] = (int32_t)bitfield_insert(~cm, (uint32_t)v[a], (uint32_t)r.dm[a]);
         int32_t rel = (int32_t)((uint32_t)v[a] - (uint32_t)r.W[a]);
         r.side[a] = fmaf((float)rel, r.inv[a], r.ts[a]);
     }
@@ -121,7 +155,40 @@ __device__ __forceinline__ int ray_step(const WorldView& w, RayState& r, int32_t
     tmin += 0.001f;
 #pragma unroll
     for (int a = 0; a < 3; a++) r.cur[a] = fmaf(tmin, r.d[a], r.o[a]);
-    return STEP_CONTINUE;
+    return true;
+}
+
+// ray_advance as a result code.
+template <bool EXACT>
+__device__ __forceinline__ int ray_step(const WorldView& w, RayState& r, int32_t v[3], uint32_t* hit_slot) {
+    bool hit;
+    if (ray_advance<EXACT>(w, r, v, hit, hit_slot)) return STEP_CONTINUE;
+    return hit ? STEP_HIT : STEP_MISS;
+}
+
+// Brick slot of voxel v, which lies in an allocated brick (a hit voxel): the
+// lookup of a descending step, repeated once after the loop so the loop does
+// not carry the slot.
+__device__ __forceinline__ uint32_t brick_slot_at(const WorldView& w, const int32_t v[3]) {
+    uint32_t ux = (uint32_t)v[0], uy = (uint32_t)v[1], uz = (uint32_t)v[2];
+    uint32_t si = (ux >> 5) | (uz >> 5) << w.shift_xz | (uy >> 5) << (2 * w.shift_xz);
+    uint4 sec = w.sec[si];
+    uint64_t m = (uint64_t)sec.x | (uint64_t)sec.y << 32;
+    uint32_t bi = __builtin_amdgcn_ubfe(ux, 3, 2) | __builtin_amdgcn_ubfe(uz, 3, 2) << 2 |
+                  __builtin_amdgcn_ubfe(uy, 3, 2) << 4;
+    return sec.z + (uint32_t)__popcll(m & ((1ull << bi) - 1ull));
+}
+
+// RayCast's loop (CpuRenderer.cpp:185-202): steps until a hit, a miss or
+// max_iters steps; true on a hit, with v the hit voxel (its slot is
+// brick_slot_at(w, v)).
+template <bool EXACT>
+__device__ __forceinline__ bool ray_trace(const WorldView& w, RayState& r, uint32_t max_iters, int32_t v[3]) {
+    bool hit = false;
+    uint32_t unused;
+    for (uint32_t it = 0; it < max_iters; it++)
+        if (!ray_advance<EXACT>(w, r, v, hit, &unused)) break;
+    return hit;
 }
 
 // Hit record fields (vrt_hit in vrt.h) of a finished ray.

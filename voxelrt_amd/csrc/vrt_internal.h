@@ -47,6 +47,10 @@ bool fault_injected(const char* entry);
 
 // ---------------------------------------------------------------- slot arena
 
+// Largest slot arena of a world.
+constexpr uint64_t kMaxSlots = 1ull << 28;
+
+
 // Free-list arena of brick slots.  Each sector owns one contiguous run of
 // popcount(mask) slots.  Slot 0 is valid (no sentinel).  Best-fit allocation,
 // growth in place when the following range is free, coalescing on free.
@@ -81,6 +85,21 @@ struct WorldView {
     uint32_t shift_xz, shift_y;          // log2 sector grid extents
 };
 
+// What the shader method reads beyond WorldView: the sector-of-sectors masks
+// (one u64 per 4x4x4 group of sectors, bit = that sector has a brick; group grid
+// of max(1, sectors / 4) per axis) and the ray-cell interaction table.
+struct ShaderView {
+    WorldView w;
+    const uint64_t* __restrict__ groups;  // [1 << (2 * gshift_xz + gshift_y)]
+    const uint64_t* __restrict__ lut;     // [8 octants * 64 cells]
+    uint32_t gshift_xz, gshift_y;
+};
+
+// Ray-cell interaction table: for direction octant oct (bit a set = positive
+// along axis a) and cell c of a 4x4x4 mask, the cells c + j * dir (j >= 0 per
+// axis) inside the mask, i.e. the cells a ray in that octant can reach from c.
+void ray_cell_lut(uint64_t table[512]);
+
 }  // namespace vrt
 
 struct vrt_world {
@@ -97,6 +116,11 @@ struct vrt_world {
     std::vector<uint32_t> h_base;
     vrt::SlotArena arena;
     uint32_t num_bricks = 0;
+    // shader method: sector-of-sectors masks and the ray-cell table
+    uint32_t gshift_xz = 0, gshift_y = 0;
+    std::vector<uint64_t> h_groups;
+    uint64_t* d_groups = nullptr;
+    uint64_t* d_lut = nullptr;
 
     vrt::WorldView view() const {
         vrt::WorldView v;
@@ -107,6 +131,15 @@ struct vrt_world {
         v.size_y = 1u << log2_y;
         v.shift_xz = shift_xz;
         v.shift_y = shift_y;
+        return v;
+    }
+    vrt::ShaderView shader_view() const {
+        vrt::ShaderView v;
+        v.w = view();
+        v.groups = d_groups;
+        v.lut = d_lut;
+        v.gshift_xz = gshift_xz;
+        v.gshift_y = gshift_y;
         return v;
     }
 };

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <bit>
 #include <cstdlib>
 #include <cstring>
@@ -180,7 +181,36 @@ static void free_world(vrt_world* w) {
     (void)hipFree(w->d_occ);
     (void)hipFree(w->d_vox);
     (void)hipFree(w->d_palette);
+    (void)hipFree(w->d_groups);
+    (void)hipFree(w->d_lut);
     delete w;
+}
+
+void vrt::ray_cell_lut(uint64_t table[512]) {
+    for (uint32_t oct = 0; oct < 8; oct++)
+        for (uint32_t c = 0; c < 64; c++) {
+            uint32_t cx = c & 3u, cz = (c >> 2) & 3u, cy = (c >> 4) & 3u;
+            // reachable coordinates along each axis: [c, 3] going up, [0, c] going down
+            auto range = [&](uint32_t v, bool up) -> uint32_t { return up ? (0xFu << v) & 0xFu : 0xFu >> (3u - v); };
+            uint32_t rx = range(cx, oct & 1u), ry = range(cy, oct & 2u), rz = range(cz, oct & 4u);
+            uint64_t m = 0;
+            for (uint32_t b = 0; b < 64; b++)
+                if ((rx >> (b & 3u) & 1u) && (rz >> ((b >> 2) & 3u) & 1u) && (ry >> (b >> 4) & 1u)) m |= 1ull << b;
+            table[oct * 64 + c] = m;
+        }
+}
+
+// Sector-of-sectors bit of sector idx, from its current mask.
+static void update_group_bit(vrt_world* w, uint32_t idx) {
+    uint32_t nxz = 1u << w->shift_xz;
+    uint32_t sx = idx & (nxz - 1u), sz = (idx >> w->shift_xz) & (nxz - 1u), sy = idx >> (2 * w->shift_xz);
+    uint32_t gm_xz = (1u << w->gshift_xz) - 1u, gm_y = (1u << w->gshift_y) - 1u;
+    uint32_t g = ((sx >> 2) & gm_xz) | ((sz >> 2) & gm_xz) << w->gshift_xz | ((sy >> 2) & gm_y) << (2 * w->gshift_xz);
+    uint64_t bit = 1ull << ((sx & 3u) | (sz & 3u) << 2 | (sy & 3u) << 4);
+    if (w->h_mask[idx])
+        w->h_groups[g] |= bit;
+    else
+        w->h_groups[g] &= ~bit;
 }
 
 // ---------------------------------------------------------------- C ABI
@@ -210,19 +240,33 @@ int vrt_world_create(const vrt_world_desc* desc, void* stream, vrt_world** out) 
         w->nsec = 1u << (2 * w->shift_xz + w->shift_y);
         uint64_t max_slots = (uint64_t)w->nsec * 64;
         if (desc->max_slots) max_slots = std::min<uint64_t>(max_slots, desc->max_slots);
-        max_slots = std::min<uint64_t>(max_slots, 1ull << 31);
+        // occupancy indices slot * 8 + cell are 32-bit in the traversal: at most 2^28 slots
+        // (154 GB of brick data)
+        max_slots = std::min<uint64_t>(max_slots, kMaxSlots);
         w->arena = SlotArena((uint32_t)max_slots);
         w->h_mask.assign(w->nsec, 0);
         w->h_base.assign(w->nsec, 0);
+        w->gshift_xz = w->shift_xz > 2 ? w->shift_xz - 2 : 0;
+        w->gshift_y = w->shift_y > 2 ? w->shift_y - 2 : 0;
+        w->h_groups.assign((size_t)1 << (2 * w->gshift_xz + w->gshift_y), 0);
+        static const std::array<uint64_t, 512> lut = [] {
+            std::array<uint64_t, 512> t;
+            ray_cell_lut(t.data());
+            return t;
+        }();
         int rc = VRT_OK;
         auto check = [&](hipError_t e, const char* what) {
             if (e != hipSuccess && rc == VRT_OK) rc = hip_fail(e, what);
         };
         check(hipMalloc(&w->d_sec, (size_t)w->nsec * 16), "hipMalloc(sector records)");
         check(hipMalloc(&w->d_palette, 256 * 8), "hipMalloc(palette)");
+        check(hipMalloc(&w->d_groups, w->h_groups.size() * 8), "hipMalloc(sector groups)");
+        check(hipMalloc(&w->d_lut, 512 * 8), "hipMalloc(ray-cell table)");
         if (rc == VRT_OK) {
             check(hipMemsetAsync(w->d_sec, 0, (size_t)w->nsec * 16, s), "hipMemsetAsync(sector records)");
             check(hipMemsetAsync(w->d_palette, 0, 256 * 8, s), "hipMemsetAsync(palette)");
+            check(hipMemsetAsync(w->d_groups, 0, w->h_groups.size() * 8, s), "hipMemsetAsync(sector groups)");
+            check(hipMemcpyAsync(w->d_lut, lut.data(), 512 * 8, hipMemcpyHostToDevice, s), "hipMemcpyAsync(ray-cell table)");
         }
         if (rc == VRT_OK && desc->initial_slots) rc = grow_slots(w, std::min(desc->initial_slots, w->arena.capacity()), s);
         if (rc == VRT_OK) check(hipStreamSynchronize(s), "hipStreamSynchronize");
@@ -342,6 +386,7 @@ int vrt_world_sync(vrt_world* w, uint32_t nsec, const uint32_t* sector_idx, cons
             w->num_bricks += (uint32_t)std::popcount(alloc_mask[i]);
             w->h_mask[idx] = alloc_mask[i];
             w->h_base[idx] = new_base[i];
+            update_group_bit(w, idx);
         }
         for (const Run& r : runs)
             for (uint32_t k = 0; k < r.count; k++) slots.push_back(r.dst + k);
@@ -364,6 +409,7 @@ int vrt_world_sync(vrt_world* w, uint32_t nsec, const uint32_t* sector_idx, cons
         for (uint32_t i = 0; i < w->nsec; i++)
             recs[i] = make_uint4((uint32_t)w->h_mask[i], (uint32_t)(w->h_mask[i] >> 32), w->h_base[i], 0u);
         VRT_HIP(hipMemcpyAsync(w->d_sec, recs.data(), (size_t)w->nsec * 16, hipMemcpyHostToDevice, s));
+        VRT_HIP(hipMemcpyAsync(w->d_groups, w->h_groups.data(), w->h_groups.size() * 8, hipMemcpyHostToDevice, s));
         hipError_t e = hipStreamSynchronize(s);
         (void)hipFree(d_slots);
         if (e != hipSuccess) return hip_fail(e, "vrt_world_sync: hipStreamSynchronize");
