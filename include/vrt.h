@@ -310,6 +310,47 @@ typedef struct vrt_frame_opts {
 int vrt_cast_frame(const vrt_world* world, const vrt_camera* cam, uint32_t max_iters, uint32_t seed,
                    vrt_hits* primary, vrt_hits* bounce, const vrt_frame_opts* opts, void* stream);
 
+/* ---------------------------------------------------------------- renderer */
+
+/*
+ * One frame of the reference GPU renderer into G-buffer images: the body of
+ * VoxelRender.comp:28-94 per pixel, the work GpuRenderer::RenderFrame
+ * (GpuRenderer.cpp:239-273) dispatches in 8x8 groups.  The primary ray is
+ * traced with the shader method's rayCast (vrt_cast_shader); on a hit the
+ * palette gives the albedo (RGB565 / (31, 63, 31), squared) and the emission
+ * (the half in bits 16..31); with max_bounces > 0 a coarse shadow ray toward the
+ * sun normalize(0.3, 0.9, -0.28) adds (1.2, 1.1, 1.0) * 5 or halves the
+ * throughput, then up to max_bounces coarse diffuse bounces (the first two also
+ * test the sun) accumulate emission; a miss adds the sky.  Deviations from the
+ * shader, all stated: rays come from vrt_camera's pinhole model (pixel centres,
+ * image row-major, tile fields ignored), the sky is a constant radiance instead
+ * of a cubemap, bounce directions take PCG samples of random_init(uvec3(px, py,
+ * frame)) instead of blue noise, bounce directions get the zero nudge of
+ * vrt_camera rays, irradiance is fp32 instead of fp16, and a ray that exhausts
+ * its cap reports the cap as its iterations.
+ * Depth is (pa * -q + pb) / q with q = dot(forward, pos / 16) (pos relative to
+ * world_origin), pa = -(f + n) / (f - n), pb = -(2 f n) / (f - n), the z row of
+ * glm::perspective (Camera.h:34); forward must be the unit view axis.
+ */
+typedef struct vrt_render_opts {
+    uint32_t max_bounces;   /* u_MaxBounces, 0..16 */
+    uint32_t frame;         /* u_FrameNo: seeds the bounce directions */
+    uint32_t ray_cell_lut;  /* u_UseAnisotropicLods: 0 or 1 */
+    uint32_t reserved;      /* 0 */
+    float sky[3];           /* sky radiance */
+    float near_z, far_z;    /* depth projection; both 0 = glim::Camera's 0.01 / 1000 */
+} vrt_render_opts;
+
+typedef struct vrt_gbuffer {
+    uint32_t* albedo_normal;  /* [height][width] RGBA8 (u_AlbedoNormalTex): albedo as unorm8 round(clamp(x) * 255),
+                                 alpha = packGNormal: (nx + 1) | (ny + 1) << 2 | (nz + 1) << 4, 21 on a miss */
+    float* irradiance;        /* [height][width][4] (u_IrradianceTex): rgb, iterations of the last traced ray */
+    float* depth;             /* [height][width] (u_DepthTex), -1 on a miss */
+} vrt_gbuffer;
+
+int vrt_render_frame(const vrt_world* world, const vrt_camera* cam, const vrt_render_opts* opts,
+                     vrt_gbuffer* out, void* stream);
+
 /* ---------------------------------------------------------------- batched */
 
 typedef enum vrt_segment_kind {

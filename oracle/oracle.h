@@ -124,6 +124,20 @@ void oracle_ray_cell_lut(uint64_t table[512]);
 void oracle_cast_shader(const oracle_world* w, const float* const o[3], const float* const d[3],
                         const int32_t world_origin[3], uint32_t begin, uint32_t end, uint32_t method,
                         uint32_t lut, uint32_t max_iters, float* t, oracle_hit* rec, float* const pos[3]);
+/* One ray of the shader method; lut = the ray-cell table or NULL; cap > 0. */
+void oracle_shader_ray(const oracle_world* w, const float o[3], const float d[3], const int32_t W[3],
+                       uint32_t method, const uint64_t* lut, uint32_t cap, float* t, oracle_hit* rec, float pos[3]);
+
+/* One frame of VoxelRender.comp:28-94 into G-buffer images, as vrt_render_frame
+ * (include/vrt.h) states it: primary rays o, d given per pixel (row-major), casts
+ * of the shader method, sun / bounce / sky shading, PCG bounce directions.
+ * sun: the unit sun direction; pa, pb: the depth projection terms
+ * -(f + n) / (f - n) and -(2 f n) / (f - n); fwd: the unit view axis. */
+void oracle_render_frame(const oracle_world* w, uint32_t width, uint32_t height, const float* const o[3],
+                         const float* const d[3], const int32_t W[3], const float fwd[3], const float sun[3],
+                         float pa, float pb, uint32_t max_bounces, uint32_t frame, uint32_t lut,
+                         const float sky[3], uint32_t* albedo_normal, float* irradiance, float* depth);
+
 /* Internal: fills w->sector_groups from w->sector_masks. */
 int oracle_world_build_groups(oracle_world* w);
 

@@ -67,6 +67,9 @@ def lib() -> C.CDLL:
         h.oracle_render_loss.argtypes = [u32, P, P, P, P]
         h.oracle_render_loss.restype = C.c_double
         h.oracle_ray_cell_lut.argtypes = [P]
+        h.oracle_render_frame.argtypes = [P, u32, u32, C.POINTER(P * 3), C.POINTER(P * 3), C.POINTER(C.c_int32 * 3),
+                                          C.POINTER(C.c_float * 3), C.POINTER(C.c_float * 3), C.c_float, C.c_float,
+                                          u32, u32, u32, C.POINTER(C.c_float * 3), P, P, P]
         h.oracle_cast_shader.argtypes = [P, C.POINTER(P * 3), C.POINTER(P * 3), C.POINTER(C.c_int32 * 3), u32, u32,
                                          u32, u32, u32, P, P, C.c_void_p]
         _lib = h
@@ -155,6 +158,24 @@ class OracleWorld:
                                  max_iters, t.ctypes.data, rec.ctypes.data,
                                  C.cast(C.byref(_p3(pos)), C.c_void_p) if with_pos else None)
         return {"t": t[begin:end], "rec": rec[begin:end], "pos": pos[:, begin:end] if with_pos else None}
+
+    def render_frame(self, width, height, o, d, world_origin, forward, sun, pa, pb, max_bounces, frame, lut, sky):
+        """G-buffer of one frame (VoxelRender.comp:28-94 over the shader method) for row-major primary rays."""
+        o = np.ascontiguousarray(o, dtype=np.float32)
+        d = np.ascontiguousarray(d, dtype=np.float32)
+        an = np.zeros((height, width), dtype=np.uint32)
+        irr = np.zeros((height, width, 4), dtype=np.float32)
+        dep = np.zeros((height, width), dtype=np.float32)
+        f3 = C.c_float * 3
+        lib().oracle_render_frame(self._h, width, height, C.byref(_p3(o)), C.byref(_p3(d)),
+                                  C.byref((C.c_int32 * 3)(*world_origin)), C.byref(f3(*forward)), C.byref(f3(*sun)),
+                                  pa, pb, max_bounces, frame, int(lut), C.byref(f3(*sky)), an.ctypes.data,
+                                  irr.ctypes.data, dep.ctypes.data)
+        return {"albedo_normal": an, "irradiance": irr, "depth": dep}
+
+    def set_palette(self, palette) -> None:
+        pal = np.ascontiguousarray(palette, dtype=np.uint64)
+        lib().oracle_world_set_palette(self._h, pal.ctypes.data)
 
     def cast_f64(self, origin, direction, max_iters=1024) -> dict:
         o = (C.c_double * 3)(*origin)

@@ -143,9 +143,9 @@ static void clip_ray(const oracle_world* w, const float o[3], const float d[3], 
     for (int a = 0; a < 3; a++) out[a] = clip ? fmaf(d[a], tmin, o[a]) : o[a];
 }
 
-static void cast_shader_one(const oracle_world* w, const float o[3], const float d[3], const int32_t W[3],
-                            uint32_t method, const uint64_t* lut, uint32_t cap, float* t_out, oracle_hit* rec,
-                            float pos[3]) {
+void oracle_shader_ray(const oracle_world* w, const float o[3], const float d[3], const int32_t W[3],
+                       uint32_t method, const uint64_t* lut, uint32_t cap, float* t_out, oracle_hit* rec,
+                       float pos[3]) {
     float start[3], org[3], inv[3], ts[3], side[3], cur[3];
     clip_ray(w, o, d, W, start);
     for (int a = 0; a < 3; a++) {
@@ -205,7 +205,7 @@ void oracle_cast_shader(const oracle_world* w, const float* const o[3], const fl
         float oi[3] = {o[0][i], o[1][i], o[2][i]};
         float di[3] = {d[0][i], d[1][i], d[2][i]};
         float p[3];
-        cast_shader_one(w, oi, di, world_origin, method, lut ? table : NULL, cap, &t[i], &rec[i], pos ? p : NULL);
+        oracle_shader_ray(w, oi, di, world_origin, method, lut ? table : NULL, cap, &t[i], &rec[i], pos ? p : NULL);
         if (pos) {
             pos[0][i] = p[0];
             pos[1][i] = p[1];

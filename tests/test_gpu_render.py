@@ -1,0 +1,86 @@
+"""GPU parity of vrt_render_frame (the GPU renderer's shading over the shader method) with the
+oracle's restatement: G-buffer images bit-exact."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from test_render_oracle import SKY, render_constants
+
+pytestmark = pytest.mark.gpu
+
+
+def random_palette(seed: int) -> np.ndarray:
+    """RGB565 colours, a few emissive materials (half emission in 0..4), fuzziness byte."""
+    rng = np.random.default_rng(seed)
+    rgb = rng.integers(0, 1 << 16, 256).astype(np.uint64)
+    emit = np.zeros(256, np.float16)
+    hot = rng.random(256) < 0.05
+    emit[hot] = rng.uniform(0, 4, int(hot.sum())).astype(np.float16)
+    pal = rgb | (emit.view(np.uint16).astype(np.uint64) << np.uint64(16))
+    pal |= rng.integers(0, 256, 256).astype(np.uint64) << np.uint64(32)
+    pal[0] = 0
+    return pal
+
+
+def render_both(vrt, world, ow, cam, bounces, frame, lut):
+    g = vrt.render_frame(world, cam, bounces, frame, lut, SKY)
+    torch.cuda.synchronize()
+    o, d = cam.rays_host()
+    sun, pa, pb = render_constants()
+    ref = ow.render_frame(cam.c.width, cam.c.height, o, d, cam.world_origin, tuple(cam.c.forward), sun, pa, pb,
+                          bounces, frame, lut, SKY)
+    return g, ref
+
+
+@pytest.mark.parametrize("bounces,lut", [(0, False), (1, False), (2, True), (4, False)])
+def test_c1_terrain_gbuffer(vrt, oracle_mod, cuda, bounces, lut):
+    from voxelrt_amd import workloads as W
+    import oracle
+
+    sc = W.C1_WORLD.build()
+    world = vrt.World.from_scene(sc)
+    ow = oracle.OracleWorld.from_stream(sc.stream())
+    pal = random_palette(5)
+    world.set_palette(pal)
+    ow.set_palette(pal)
+    cam = W.c1_camera(tile=(1, 1), width=200, height=120)
+    g, ref = render_both(vrt, world, ow, cam, bounces, 9, lut)
+    an = g.albedo_normal.cpu().numpy().view(np.uint32)
+    assert np.array_equal(an, ref["albedo_normal"])
+    irr = g.irradiance.cpu().numpy()
+    bad = np.nonzero(np.any(irr.view(np.uint32) != ref["irradiance"].view(np.uint32), axis=2))
+    assert bad[0].size == 0, f"{bad[0].size} pixels differ; first {bad[0][0], bad[1][0]}: " \
+                             f"{irr[bad[0][0], bad[1][0]]} vs {ref['irradiance'][bad[0][0], bad[1][0]]}"
+    assert np.array_equal(g.depth.cpu().numpy().view(np.uint32), ref["depth"].view(np.uint32))
+    hit = ref["depth"] != -1.0
+    assert 0.2 < hit.mean() < 1.0
+
+
+def test_gbuffer_odd_size_and_errors(vrt, oracle_mod, cuda):
+    """An image that is not a multiple of the 8x8 tile, and argument errors."""
+    import ctypes as C
+
+    from voxelrt_amd import _lib as L
+    from voxelrt_amd import workloads as W
+    import oracle
+
+    sc = W.C1_WORLD.build()
+    world = vrt.World.from_scene(sc)
+    ow = oracle.OracleWorld.from_stream(sc.stream())
+    pal = random_palette(6)
+    world.set_palette(pal)
+    ow.set_palette(pal)
+    cam = W.c1_camera(tile=(1, 1), width=37, height=21)
+    g, ref = render_both(vrt, world, ow, cam, 1, 2, True)
+    assert np.array_equal(g.albedo_normal.cpu().numpy().view(np.uint32), ref["albedo_normal"])
+    assert np.array_equal(g.irradiance.cpu().numpy().view(np.uint32), ref["irradiance"].view(np.uint32))
+    assert np.array_equal(g.depth.cpu().numpy().view(np.uint32), ref["depth"].view(np.uint32))
+    o = L.RenderOpts(17, 0, 0, 0, (C.c_float * 3)(*SKY), 0.0, 0.0)
+    c = g.to_c()
+    assert vrt.lib().vrt_render_frame(world.handle, C.byref(cam.c), C.byref(o), C.byref(c), None) == \
+        L.VRT_ERR_INVALID_ARGUMENT
+    o = L.RenderOpts(1, 0, 0, 0, (C.c_float * 3)(*SKY), 5.0, 1.0)
+    assert vrt.lib().vrt_render_frame(world.handle, C.byref(cam.c), C.byref(o), C.byref(c), None) == \
+        L.VRT_ERR_INVALID_ARGUMENT

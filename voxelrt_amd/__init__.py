@@ -16,6 +16,8 @@ from .api import (  # noqa: F401
     new_counters,
     bounce,
     cast_frame,
+    render_frame,
+    GBufferImages,
     Batch,
     BatchPlan,
     autotune,
@@ -27,6 +29,7 @@ from .api import (  # noqa: F401
 
 __all__ = [
     "VrtError", "CapacityError", "lib", "LIB_PATH", "Scene", "World", "RayBatch", "HitBatch", "HitResult",
-    "CameraSpec", "incoherent_rays_host", "new_counters", "bounce", "cast_frame", "Batch", "BatchPlan", "autotune", "render",
+    "CameraSpec", "incoherent_rays_host", "new_counters", "bounce", "cast_frame", "render_frame",
+    "GBufferImages", "Batch", "BatchPlan", "autotune", "render",
     "render_saved", "render_backward", "sun_direction",
 ]

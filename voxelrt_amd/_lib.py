@@ -103,6 +103,15 @@ class ShaderOpts(C.Structure):
     _fields_ = [("method", u32), ("ray_cell_lut", u32), ("max_iters", u32), ("reserved", u32)]
 
 
+class RenderOpts(C.Structure):
+    _fields_ = [("max_bounces", u32), ("frame", u32), ("ray_cell_lut", u32), ("reserved", u32), ("sky", f32 * 3),
+                ("near_z", f32), ("far_z", f32)]
+
+
+class GBuffer(C.Structure):
+    _fields_ = [("albedo_normal", P), ("irradiance", P), ("depth", P)]
+
+
 class FrameOpts(C.Structure):
     _fields_ = [("schedule", u32), ("refill", u32), ("chunk", u32), ("counters", P)]
 
@@ -158,6 +167,7 @@ _SIGNATURES = {
     "vrt_bounce_generate": (C.c_int, [C.POINTER(Rays), C.POINTER(Hits), u32, u32, C.POINTER(RayBuffers), P, P, P, P]),
     "vrt_camera_rays": (C.c_int, [C.POINTER(Camera), C.POINTER(RayBuffers), P]),
     "vrt_cast_shader": (C.c_int, [P, C.POINTER(Rays), C.POINTER(ShaderOpts), C.POINTER(Hits), P]),
+    "vrt_render_frame": (C.c_int, [P, C.POINTER(Camera), C.POINTER(RenderOpts), C.POINTER(GBuffer), P]),
     "vrt_cast_frame": (C.c_int, [P, C.POINTER(Camera), u32, u32, C.POINTER(Hits), C.POINTER(Hits),
                                  C.POINTER(FrameOpts), P]),
     "vrt_batch_plan_create": (C.c_int, [P, C.POINTER(Batch), C.POINTER(BatchTuning), P, C.POINTER(P)]),

@@ -342,6 +342,30 @@ def cast_frame(world: "World", cam: "CameraSpec", max_iters: int = 128, seed: in
     return hp, hb
 
 
+class GBufferImages:
+    """G-buffer images of vrt_render_frame on the device: albedo_normal [h, w] int32 (RGBA8 bytes),
+    irradiance [h, w, 4] float32, depth [h, w] float32."""
+
+    def __init__(self, width: int, height: int, device="cuda"):
+        self.albedo_normal = torch.empty((height, width), dtype=torch.int32, device=device)
+        self.irradiance = torch.empty((height, width, 4), dtype=torch.float32, device=device)
+        self.depth = torch.empty((height, width), dtype=torch.float32, device=device)
+
+    def to_c(self) -> L.GBuffer:
+        return L.GBuffer(self.albedo_normal.data_ptr(), self.irradiance.data_ptr(), self.depth.data_ptr())
+
+
+def render_frame(world: "World", cam: "CameraSpec", max_bounces: int = 1, frame: int = 0,
+                 ray_cell_lut: bool = False, sky=(0.6, 0.75, 1.0), near_z: float = 0.0, far_z: float = 0.0,
+                 out: GBufferImages | None = None, stream=None) -> GBufferImages:
+    """vrt_render_frame: one frame of the GPU renderer (VoxelRender.comp:28-94) into G-buffer images."""
+    g = out if out is not None else GBufferImages(cam.c.width, cam.c.height)
+    o = L.RenderOpts(max_bounces, frame, int(ray_cell_lut), 0, (C.c_float * 3)(*sky), near_z, far_z)
+    c = g.to_c()
+    L.check(L.lib().vrt_render_frame(world.handle, C.byref(cam.c), C.byref(o), C.byref(c), _stream_handle(stream)))
+    return g
+
+
 def new_counters(device="cuda") -> torch.Tensor:
     """Zeroed vrt_cast_counters on the device (rays, iters, descents, hits)."""
     return torch.zeros(4, dtype=torch.int64, device=device)

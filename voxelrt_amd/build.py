@@ -19,7 +19,7 @@ BUILD = PKG / "_build"
 LIB = PKG / "libvoxelrt_hip.so"
 ORACLE_DIR = ROOT / "oracle"
 
-SOURCES = ["world.hip", "cast.hip", "rays.hip", "frame.hip", "shader.hip", "scene.hip", "batch.hip", "diff.hip"]
+SOURCES = ["world.hip", "cast.hip", "rays.hip", "frame.hip", "shader.hip", "render.hip", "scene.hip", "batch.hip", "diff.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("VRT_OFFLOAD_ARCH", "gfx950")

@@ -97,13 +97,10 @@ __host__ __device__ inline void tile_pixel(uint32_t i, uint32_t width, uint32_t 
     py = ty * th + in / tw;
 }
 
-// Diffuse bounce of one hit (CpuRenderer.cpp:389-392, VoxelRender.comp:63-71):
-// origin = pos + N * 0.01, direction = normalize(N + s), s uniform on the unit
-// sphere from two PCG samples; N + s = 0 falls back to N.
-__host__ __device__ inline void bounce_ray(uint32_t info, const float pos[3], uint32_t i, uint32_t width,
-                                           uint32_t seed, float o[3], float d[3]) {
-    float N[3] = {normal_component(info, 0), normal_component(info, 1), normal_component(info, 2)};
-    uint32_t st = pcg_init(i % width, i / width, seed);
+// normalize(N + s) for s uniform on the unit sphere from the next two PCG
+// samples of st (random_dir, RandomGen.glsl:37-48, with PCG samples in place of
+// the blue-noise texture); N + s = 0 falls back to N.
+__host__ __device__ inline void cosine_dir(const float N[3], uint32_t& st, float dd[3]) {
     float u1 = pcg_unit(st);
     float u2 = pcg_unit(st);
     float y = u1 * 2.0f - 1.0f;
@@ -112,17 +109,56 @@ __host__ __device__ inline void bounce_ray(uint32_t info, const float pos[3], ui
     float sy = sqrtf(1.0f - y * y);
     float v0 = N[0] + sn * sy, v1 = N[1] + y, v2 = N[2] + cs * sy;
     float len2 = v0 * v0 + v1 * v1 + v2 * v2;
-    float dd[3];
     if (len2 == 0.0f) {
         dd[0] = N[0]; dd[1] = N[1]; dd[2] = N[2];
     } else {
         float il = 1.0f / sqrtf(len2);
         dd[0] = v0 * il; dd[1] = v1 * il; dd[2] = v2 * il;
     }
+}
+
+// Diffuse bounce of one hit (CpuRenderer.cpp:389-392, VoxelRender.comp:63-71):
+// origin = pos + N * 0.01, direction = normalize(N + s), s uniform on the unit
+// sphere from two PCG samples; N + s = 0 falls back to N.
+__host__ __device__ inline void bounce_ray(uint32_t info, const float pos[3], uint32_t i, uint32_t width,
+                                           uint32_t seed, float o[3], float d[3]) {
+    float N[3] = {normal_component(info, 0), normal_component(info, 1), normal_component(info, 2)};
+    uint32_t st = pcg_init(i % width, i / width, seed);
+    float dd[3];
+    cosine_dir(N, st, dd);
     for (int a = 0; a < 3; a++) {
         o[a] = pos[a] + N[a] * 0.01f;
         d[a] = nudge_zero(dd[a]);
     }
 }
+
+// IEEE binary16 bits -> float (exact).
+__host__ __device__ inline float half_to_float(uint32_t h) {
+    uint32_t sign = (h & 0x8000u) << 16, e = (h >> 10) & 0x1Fu, m = h & 0x3FFu;
+    float mag;
+    if (e == 0x1Fu) {
+        uint32_t bits = 0x7F800000u | (m << 13);
+        mag = __builtin_bit_cast(float, bits);
+    } else if (e == 0) {
+        mag = (float)m * 5.9604644775390625e-8f;  // m * 2^-24
+    } else {
+        uint32_t bits = ((e + 112u) << 23) | (m << 13);
+        mag = __builtin_bit_cast(float, bits);
+    }
+    return __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, mag) | sign);
+}
+
+// Palette entry decode (Material::GetEncoded, VoxelMap.h:27-41; VoxelMap.glsl:78-85):
+// colour = (RGB565 / (31, 63, 31))^2, emission = the half in bits 16..31.
+__host__ __device__ inline void material_color(uint64_t entry, float rgb[3]) {
+    uint32_t x = (uint32_t)entry;
+    float r = (float)((x >> 11) & 31u) * (1.0f / 31.0f);
+    float g = (float)((x >> 5) & 63u) * (1.0f / 63.0f);
+    float b = (float)(x & 31u) * (1.0f / 31.0f);
+    rgb[0] = r * r;
+    rgb[1] = g * g;
+    rgb[2] = b * b;
+}
+__host__ __device__ inline float material_emission(uint64_t entry) { return half_to_float((uint32_t)entry >> 16); }
 
 }  // namespace vrt

@@ -52,8 +52,8 @@ C2_WORLD = TerrainSpec(10, 10, seed=2, base=32.0, amp=300.0, frequency=1 / 256, 
 C5_WORLD = TerrainSpec(9, 9, seed=5, base=64.0, amp=256.0, frequency=1 / 128)
 
 
-def c1_camera(tile=(8, 8)) -> CameraSpec:
-    return CameraSpec((128.37, 200.61, 12.29), yaw_deg=8.0, pitch_deg=-28.0, width=640, height=360, tile=tile)
+def c1_camera(tile=(8, 8), width=640, height=360) -> CameraSpec:
+    return CameraSpec((128.37, 200.61, 12.29), yaw_deg=8.0, pitch_deg=-28.0, width=width, height=height, tile=tile)
 
 
 def c2_camera(tile=(8, 8), width=1920, height=1080) -> CameraSpec:
