@@ -73,6 +73,35 @@ int vrt_scene_from_dense(uint32_t log2_size_xz, uint32_t log2_size_y, const uint
 int vrt_scene_get(const vrt_scene* scene, vrt_sector_stream* out);
 /* Voxel at absolute coordinates (0 outside the world). */
 uint8_t vrt_scene_voxel(const vrt_scene* scene, int32_t x, int32_t y, int32_t z);
+
+/*
+ * The reference's world file, "cvox 0004" (VoxelMap::Deserialize / Serialize,
+ * src/VoxelRT/VoxelMap.cpp:207-274; zstd blobs of BinaryIO.cpp:8-60): a palette
+ * of 256 Material records and the sectors of a VoxelMap, indexed by
+ * WorldSectorIndexer<12,8,signed> (VoxelMap.h:100, signed x and z in 12 bits,
+ * y in 8).  Loading places world sector s at scene sector s - sector_offset
+ * (NULL = 0) of a scene of the given extents and skips (and counts) sectors
+ * outside it; the palette becomes the scene's, encoded as Material::GetEncoded
+ * (VoxelMap.h:27-41: RGB565 | half(emission) << 16 | fuzziness << 32; the half
+ * rounds to nearest even), and vrt_world_from_scene uploads it.  Saving writes a
+ * scene in the same layout (scene sector s at world sector s + sector_offset,
+ * increasing index, packs of at most 16 MiB plus a final one) with the given
+ * 256 x 16-byte Material records (NULL = zeros).  zstd is libzstd.so.1, opened
+ * at run time; without it both calls fail with VRT_ERR_STATE.
+ */
+typedef struct vrt_cvox_info {
+    uint32_t sectors_in_file;
+    uint32_t sectors_loaded;
+    uint32_t sectors_outside;   /* non-empty sectors outside the scene's extents */
+    uint64_t bricks_loaded;
+} vrt_cvox_info;
+
+int vrt_scene_load_cvox(const char* path, uint32_t log2_size_xz, uint32_t log2_size_y,
+                        const int32_t sector_offset[3], vrt_scene** out, vrt_cvox_info* info);
+int vrt_scene_save_cvox(const vrt_scene* scene, const int32_t sector_offset[3], const uint8_t* materials,
+                        const char* path);
+/* The scene's encoded palette (a loaded file's); VRT_ERR_STATE for generated scenes. */
+int vrt_scene_get_palette(const vrt_scene* scene, uint64_t palette[256]);
 void vrt_scene_destroy(vrt_scene* scene);
 
 /* Convenience: vrt_world_create + vrt_world_sync of the whole scene. */

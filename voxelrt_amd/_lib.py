@@ -103,6 +103,11 @@ class ShaderOpts(C.Structure):
     _fields_ = [("method", u32), ("ray_cell_lut", u32), ("max_iters", u32), ("reserved", u32)]
 
 
+class CvoxInfo(C.Structure):
+    _fields_ = [("sectors_in_file", u32), ("sectors_loaded", u32), ("sectors_outside", u32),
+                ("bricks_loaded", u64)]
+
+
 class RenderOpts(C.Structure):
     _fields_ = [("max_bounces", u32), ("frame", u32), ("ray_cell_lut", u32), ("reserved", u32), ("sky", f32 * 3),
                 ("near_z", f32), ("far_z", f32)]
@@ -184,6 +189,9 @@ _SIGNATURES = {
     "vrt_scene_get": (C.c_int, [P, C.POINTER(SectorStream)]),
     "vrt_scene_voxel": (C.c_uint8, [P, i32, i32, i32]),
     "vrt_scene_destroy": (None, [P]),
+    "vrt_scene_load_cvox": (C.c_int, [C.c_char_p, u32, u32, C.POINTER(i32 * 3), C.POINTER(P), C.POINTER(CvoxInfo)]),
+    "vrt_scene_save_cvox": (C.c_int, [P, C.POINTER(i32 * 3), P, C.c_char_p]),
+    "vrt_scene_get_palette": (C.c_int, [P, P]),
     "vrt_world_from_scene": (C.c_int, [P, P, C.POINTER(P)]),
     "vrt_camera_look": (C.c_int, [C.POINTER(C.c_double * 3), C.c_double, C.c_double, C.c_double, u32, u32, u32, u32,
                                   C.POINTER(Camera)]),

@@ -73,6 +73,31 @@ class Scene:
                                              v.ctypes.data, C.byref(out)))
         return Scene(out.value)
 
+    @staticmethod
+    def load_cvox(path: str, log2_xz: int, log2_y: int, sector_offset=(0, 0, 0)) -> tuple["Scene", dict]:
+        """VoxelMap::Deserialize (VoxelMap.cpp:211-244): world sector s lands at scene sector
+        s - sector_offset.  Returns (scene, info dict)."""
+        out = C.c_void_p()
+        info = L.CvoxInfo()
+        L.check(L.lib().vrt_scene_load_cvox(str(path).encode(), log2_xz, log2_y,
+                                            C.byref((C.c_int32 * 3)(*sector_offset)), C.byref(out), C.byref(info)))
+        return Scene(out.value), {k: getattr(info, k) for k, _ in L.CvoxInfo._fields_}
+
+    def save_cvox(self, path: str, sector_offset=(0, 0, 0), materials: np.ndarray | None = None) -> None:
+        """VoxelMap::Serialize (VoxelMap.cpp:245-274); materials: 256 x 16-byte Material records."""
+        m = None
+        if materials is not None:
+            m = np.ascontiguousarray(materials, dtype=np.uint8).reshape(-1)
+            assert m.size == 4096
+        L.check(L.lib().vrt_scene_save_cvox(self._h, C.byref((C.c_int32 * 3)(*sector_offset)),
+                                            m.ctypes.data if m is not None else None, str(path).encode()))
+
+    def palette(self) -> np.ndarray:
+        """Encoded palette of a loaded file (Material::GetEncoded per entry)."""
+        pal = np.zeros(256, dtype=np.uint64)
+        L.check(L.lib().vrt_scene_get_palette(self._h, pal.ctypes.data))
+        return pal
+
     def stream(self) -> dict:
         """Sector stream views into the scene's memory; the dict keeps the scene alive."""
         s = L.SectorStream()

@@ -19,7 +19,7 @@ BUILD = PKG / "_build"
 LIB = PKG / "libvoxelrt_hip.so"
 ORACLE_DIR = ROOT / "oracle"
 
-SOURCES = ["world.hip", "cast.hip", "rays.hip", "frame.hip", "shader.hip", "render.hip", "scene.hip", "batch.hip", "diff.hip"]
+SOURCES = ["world.hip", "cast.hip", "rays.hip", "frame.hip", "shader.hip", "render.hip", "scene.hip", "cvox.hip", "batch.hip", "diff.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("VRT_OFFLOAD_ARCH", "gfx950")
@@ -58,7 +58,7 @@ def build_library(verbose: bool = True) -> Path:
     with cf.ThreadPoolExecutor(max_workers=workers) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs), "-lpthread"]
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs), "-lpthread", "-ldl"]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"link failed:\n{' '.join(cmd)}\n{res.stdout}\n{res.stderr}")

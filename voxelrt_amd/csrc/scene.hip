@@ -18,6 +18,7 @@
 #include "vrt_internal.h"
 #include "vrt_math.h"
 #include "vrt_scene.h"
+#include "vrt_scene_impl.h"
 
 namespace {
 
@@ -122,16 +123,6 @@ void parallel_for(uint32_t n, uint32_t threads, F&& f) {
 
 // ---------------------------------------------------------------- scene object
 
-struct vrt_scene {
-    uint32_t log2_xz = 0, log2_y = 0;
-    uint32_t shift_xz = 0, shift_y = 0;
-    std::vector<uint32_t> sector_idx;
-    std::vector<uint64_t> alloc_mask;
-    std::vector<uint8_t> bricks;
-    std::vector<uint64_t> brick_offset;   // first brick of each listed sector
-    std::vector<int32_t> sector_lookup;   // sector index -> position in the lists, -1 if absent
-    uint64_t solid = 0;
-};
 
 namespace {
 
@@ -393,7 +384,10 @@ int vrt_world_from_scene(const vrt_scene* sc, void* stream, vrt_world** out) {
         return fail(rc, msg);
     }
     uint64_t pal[256];
-    for (uint32_t i = 0; i < 256; i++) pal[i] = (uint64_t)i * 0x01010101ull;  // placeholder colours by id
+    if (sc->has_palette)
+        std::memcpy(pal, sc->palette, sizeof(pal));
+    else
+        for (uint32_t i = 0; i < 256; i++) pal[i] = (uint64_t)i * 0x01010101ull;  // placeholder colours by id
     return vrt_world_set_palette(*out, pal, stream);
 }
 
