@@ -97,10 +97,11 @@ __global__ __launch_bounds__(256) void cast_grid_kernel(WorldView w, RaysIn r, H
 // part is drained, from the other parts in turn.  The counters live on separate
 // 256-byte lines, so claims from different XCDs do not serialize.
 // Each wave keeps 64 ray slots.  A finished ray parks its result in registers;
-// once at least refill_min lanes are free (or all of them), the wave writes
-// the parked results and loads new rays into the free lanes from its pool of
-// consecutive ray indices.  Retire and refill therefore cost one divergent
-// block per refill_min rays rather than one per ray.
+// once at least refill_min lanes are free (or all of them), the wave writes the
+// parked results and starts new rays in the free lanes.  Every lane also holds
+// the origin and direction of its next ray, loaded at the previous refill, so a
+// refill starts rays without waiting on memory; the lanes that consumed their
+// prefetched ray issue the next loads and go on stepping while they arrive.
 template <bool COUNT>
 __global__ __launch_bounds__(256) void cast_persistent_kernel(WorldView w, RaysIn r, HitsOut h, uint32_t max_iters,
                                                               uint32_t chunk, uint32_t refill_min, ChunkQueue cq,
@@ -116,24 +117,51 @@ __global__ __launch_bounds__(256) void cast_persistent_kernel(WorldView w, RaysI
         pool_next = min(first * chunk, n);  // chunks past a device-side count are empty
         pool_end = min(pool_next + chunk, n);
     }
-    bool exhausted = false;                // wave-uniform: every part is drained
-    bool tame = true;                      // wave-uniform: every ray in the wave is tame
-    bool has = false;                      // the lane holds a ray (running or parked)
-    bool done = false;                     // the lane's ray has finished, result parked
-    int res = STEP_CONTINUE;
-    uint32_t idx = 0, slot = 0;
+    bool exhausted = false;  // wave-uniform: every part is drained
+    // Gives the lanes in `need` consecutive ray indices from the wave's pool (one chunk
+    // claim at most); a lane left over gets ~0u and asks again at the next refill.
+    auto take = [&](bool need) -> uint32_t {
+        uint64_t m = __ballot(need);
+        if (m == 0) return ~0u;
+        if (pool_next == pool_end && !exhausted) {
+            uint32_t c = claim_chunk(cq, home, probe);
+            if (c == ~0u) {
+                exhausted = true;
+            } else {
+                pool_next = min(c * chunk, n);
+                pool_end = min(pool_next + chunk, n);
+            }
+        }
+        uint32_t avail = pool_end - pool_next;
+        uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+        uint32_t got = need && rank < avail ? pool_next + rank : ~0u;
+        pool_next += min(avail, (uint32_t)__popcll(m));
+        return got;
+    };
+    // the lane's next ray
+    uint32_t pf = take(true);
+    float po[3] = {0.0f, 0.0f, 0.0f}, pd[3] = {1.0f, 1.0f, 1.0f};
+    if (pf != ~0u) {
+        po[0] = r.ox[pf]; po[1] = r.oy[pf]; po[2] = r.oz[pf];
+        pd[0] = r.dx[pf]; pd[1] = r.dy[pf]; pd[2] = r.dz[pf];
+    }
+    bool more = true;       // wave-uniform: some lane holds a next ray or the pool may refill
+    bool tame = true;       // wave-uniform: every running ray is tame
+    bool running = false;   // the lane is tracing a ray
+    bool parked = false;    // the lane's ray finished; its result is not written yet
+    bool hit = false;
+    uint32_t idx = 0;
     int32_t v[3] = {0, 0, 0};
     RayState st;
     unsigned long long c_rays = 0, c_iters = 0, c_desc = 0, c_hits = 0;
     for (;;) {
-        uint64_t freem = __ballot(!has || done);
-        uint32_t nfree = (uint32_t)__popcll(freem);
-        if (nfree >= refill_min) {
-            if (has && done) {
+        uint32_t nfree = (uint32_t)__popcll(__ballot(!running));
+        if (nfree == 64 || (nfree >= refill_min && more)) {
+            if (parked) {
                 float t;
                 int4 rec;
-                if (res == STEP_HIT)
-                    ray_hit_record(w, st, v, slot, t, rec);
+                if (hit)
+                    ray_hit_record(w, st, v, brick_slot_at(w, v), t, rec);
                 else
                     ray_miss_record(st, t, rec);
                 store_result(h, idx, st, t, rec);
@@ -141,51 +169,36 @@ __global__ __launch_bounds__(256) void cast_persistent_kernel(WorldView w, RaysI
                     c_rays++;
                     c_iters += st.iters;
                     c_desc += st.descents;
-                    c_hits += (rec.w & 0xFF) != 0;
+                    c_hits += hit ? 1ull : 0ull;
                 }
-                has = false;
+                parked = false;
             }
-            if (pool_next == pool_end && !exhausted) {
-                uint32_t c = claim_chunk(cq, home, probe);
-                if (c == ~0u) {
-                    exhausted = true;
-                } else {
-                    pool_next = min(c * chunk, n);
-                    pool_end = min(pool_next + chunk, n);
-                }
+            if (!running && pf != ~0u) {
+                idx = pf;
+                ray_setup(st, po[0], po[1], po[2], pd[0], pd[1], pd[2], r.W[0], r.W[1], r.W[2]);
+                running = true;
+                hit = false;
+                pf = ~0u;
             }
-            uint32_t avail = pool_end - pool_next;
-            if (!has) {
-                uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(freem >> 32),
-                                                          __builtin_amdgcn_mbcnt_lo((uint32_t)freem, 0u));
-                if (rank < avail) {
-                    idx = pool_next + rank;
-                    ray_setup(st, r.ox[idx], r.oy[idx], r.oz[idx], r.dx[idx], r.dy[idx], r.dz[idx], r.W[0], r.W[1],
-                              r.W[2]);
-                    has = true;
-                    done = false;
-                }
+            uint32_t next = take(pf == ~0u);
+            if (pf == ~0u && next != ~0u) {
+                pf = next;
+                po[0] = r.ox[pf]; po[1] = r.oy[pf]; po[2] = r.oz[pf];
+                pd[0] = r.dx[pf]; pd[1] = r.dy[pf]; pd[2] = r.dz[pf];
             }
-            pool_next += min(avail, nfree);
-            tame = __ballot(has && !ray_is_tame(st)) == 0;
-            if (!__any(has)) {
-                if (exhausted) break;
+            more = __any(pf != ~0u) || !exhausted;
+            tame = !__any(running && !ray_is_tame(st));
+            if (!__any(running)) {
+                if (!more) break;
                 continue;
             }
         }
-        if (has && !done) {
-            // two steps per check of the free-lane count
-#pragma unroll
-            for (int k = 0; k < 2; k++) {
-                if (!done) {
-                    if (st.iters >= max_iters)
-                        res = STEP_MISS;
-                    else if (tame)
-                        res = ray_step<false>(w, st, v, &slot);
-                    else
-                        res = ray_step<true>(w, st, v, &slot);
-                    done = res != STEP_CONTINUE;
-                }
+        if (running) {
+            uint32_t unused;
+            bool cont = tame ? ray_advance<false>(w, st, v, hit, &unused) : ray_advance<true>(w, st, v, hit, &unused);
+            if (!cont || st.iters >= max_iters) {
+                running = false;
+                parked = true;
             }
         }
     }
