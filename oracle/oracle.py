@@ -71,7 +71,7 @@ def lib() -> C.CDLL:
                                           C.POINTER(C.c_float * 3), C.POINTER(C.c_float * 3), C.c_float, C.c_float,
                                           u32, u32, u32, C.POINTER(C.c_float * 3), P, P, P]
         h.oracle_cast_shader.argtypes = [P, C.POINTER(P * 3), C.POINTER(P * 3), C.POINTER(C.c_int32 * 3), u32, u32,
-                                         u32, u32, u32, P, P, C.c_void_p]
+                                         u32, u32, u32, P, P, C.POINTER(P * 3)]
         _lib = h
     return _lib
 
@@ -155,8 +155,7 @@ class OracleWorld:
         W = (C.c_int32 * 3)(*world_origin)
         m = {"full": 0, "coarse": 1}[method]
         lib().oracle_cast_shader(self._h, C.byref(_p3(o)), C.byref(_p3(d)), C.byref(W), begin, end, m, int(lut),
-                                 max_iters, t.ctypes.data, rec.ctypes.data,
-                                 C.cast(C.byref(_p3(pos)), C.c_void_p) if with_pos else None)
+                                 max_iters, t.ctypes.data, rec.ctypes.data, C.byref(_p3(pos)) if with_pos else None)
         return {"t": t[begin:end], "rec": rec[begin:end], "pos": pos[:, begin:end] if with_pos else None}
 
     def render_frame(self, width, height, o, d, world_origin, forward, sun, pa, pb, max_bounces, frame, lut, sky):

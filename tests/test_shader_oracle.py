@@ -79,8 +79,13 @@ def test_full_method_first_hit_matches_fp64_dda(oracle_mod, lut):
     iters = []
     for _ in range(4):
         o, d, W = camera_rays(rng, vox, 600)
-        r = ow.cast_shader(o, d, W, "full", lut)
+        r = ow.cast_shader(o, d, W, "full", lut, with_pos=True)
         rec = r["rec"]
+        assert np.array_equal(rec, ow.cast_shader(o, d, W, "full", lut)["rec"])
+        hits_i = (rec[:, 3] & 0xFF) != 0
+        # the reported position lies on the hit voxel (it is the point just past its entry face)
+        p = r["pos"][:, hits_i].astype(np.float64) + np.array(W, np.float64)[:, None]
+        assert np.all(np.abs(p - (rec[hits_i, :3].T + 0.5)) <= 0.5 + 1e-3)
         iters.append((rec[:, 3] >> 16) & 0xFFFF)
         for i in range(o.shape[1]):
             h, v, t, gap = ow.dda(o[:, i].astype(np.float64), d[:, i].astype(np.float64), W, 4096)
