@@ -165,8 +165,11 @@ typedef enum vrt_cast_schedule {
     VRT_SCHED_AUTO = 0,        /* pick from the autotuner cache, else the default */
     VRT_SCHED_GRID = 1,        /* one ray per lane, one lane per ray, grid covers n */
     VRT_SCHED_PERSISTENT = 2,  /* persistent waves pull ray chunks from a queue and refill idle lanes */
-    VRT_SCHED_MULTIPASS = 3    /* passes of pass_iters lookups; rays still running are packed into
+    VRT_SCHED_MULTIPASS = 3,   /* passes of pass_iters lookups; rays still running are packed into
                                   dense waves again for the next pass (long-tailed ray sets) */
+    VRT_SCHED_REPACK = 4       /* workgroups of 256 rays; every pass_iters lookups the rays still running
+                                  are packed into the workgroup's first waves through LDS, so waves stay
+                                  dense (long-tailed ray sets such as diffuse bounces) */
 } vrt_cast_schedule;
 
 typedef struct vrt_cast_opts {
@@ -177,7 +180,7 @@ typedef struct vrt_cast_opts {
     vrt_cast_counters* counters;/* optional device pointer, accumulated (not cleared) */
     const uint32_t* count;      /* optional device pointer: rays to trace, read when the kernel
                                    runs (rays->n is then the capacity and the upper bound) */
-    uint32_t pass_iters;        /* multipass: lookups per pass (0 = 16) */
+    uint32_t pass_iters;        /* multipass: lookups per pass; repack: lookups between repacks (0 = 16) */
     uint32_t passes;            /* multipass: number of passes, the last one runs to max_iters (0 = 5) */
     void* scratch;              /* multipass: device scratch of vrt_cast_scratch_bytes(n) bytes; NULL = a
                                    per-device buffer owned by the library (multipass casts on one device

@@ -14,8 +14,9 @@ pytestmark = pytest.mark.gpu
 
 GOLDEN = Path(__file__).resolve().parent / "golden"
 # (schedule, chunk or lookups per pass): grid; persistent with 64/256/1024-ray chunks;
-# multipass with the default 16 lookups x 5 passes and with 4 lookups x 8 passes
-SCHEDULES = [(1, 0), (2, 64), (2, 256), (2, 1024), (3, 0), (3, 4)]
+# multipass with the default 16 lookups x 5 passes and with 4 lookups x 8 passes;
+# repack every 16 (default) and every 3 lookups
+SCHEDULES = [(1, 0), (2, 64), (2, 256), (2, 1024), (3, 0), (3, 4), (4, 0), (4, 3)]
 
 
 def gpu_cast(world, o, d, W, max_iters=128, schedule=1, chunk=0, with_pos=False, counters=None):
@@ -25,6 +26,8 @@ def gpu_cast(world, o, d, W, max_iters=128, schedule=1, chunk=0, with_pos=False,
     if schedule == 3:
         h = world.cast(rb, max_iters, with_pos=with_pos, schedule=3, pass_iters=chunk, passes=8 if chunk else 0,
                        counters=counters)
+    elif schedule == 4:
+        h = world.cast(rb, max_iters, with_pos=with_pos, schedule=4, pass_iters=chunk, counters=counters)
     else:
         h = world.cast(rb, max_iters, with_pos=with_pos, schedule=schedule, chunk=chunk, counters=counters)
     torch.cuda.synchronize()

@@ -205,6 +205,147 @@ __global__ __launch_bounds__(256) void cast_persistent_kernel(WorldView w, RaysI
     if constexpr (COUNT) flush_counters(ctr, c_rays, c_iters, c_desc, c_hits);
 }
 
+// ---------------------------------------------------------------- REPACK schedule
+
+// A wave runs until its slowest ray.  Here a workgroup owns 256 consecutive rays
+// and traces them in segments of `seg` lookups; after each segment the rays still
+// running are written to LDS and read back densely into the workgroup's first
+// waves, so the other waves idle at the barrier and their issue slots go to other
+// workgroups.  A ray's state survives the move bit for bit: origin, direction,
+// 1/d and sideDist are stored; tStart, the corner selector and the position are
+// recomputed with the operations that produced them (the position from the exact
+// minimum, which equals the fast one here).  Finished rays leave their results in
+// LDS; the workgroup writes all 256 results, coalesced, at the end.
+struct RepackLds {
+    float o[3][256], d[3][256], inv[3][256], side[3][256];
+    uint32_t it[256];    // lookups | descents << 16
+    uint32_t slot[256];  // the ray's position in the workgroup
+    int4 rec[256];
+    float t[256];
+    float pos[3][256];
+    uint32_t wave_cnt[4];
+};
+
+// Up to seg lookups of one ray; true when it finished (hit, left the world or reached the cap).
+template <bool EXACT>
+__device__ __forceinline__ bool trace_segment(const WorldView& w, RayState& st, uint32_t seg, uint32_t max_iters,
+                                              int32_t v[3], bool& hit) {
+    uint32_t unused;
+    for (uint32_t k = 0; k < seg; k++)
+        if (!ray_advance<EXACT>(w, st, v, hit, &unused) || st.iters >= max_iters) return true;
+    return false;
+}
+
+template <bool COUNT>
+__global__ __launch_bounds__(256) void cast_repack_kernel(WorldView w, RaysIn r, HitsOut h, uint32_t max_iters,
+                                                          uint32_t seg, vrt_cast_counters* ctr) {
+    __shared__ RepackLds L;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    const uint32_t n = ray_count(r);
+    const uint32_t base = blockIdx.x * 256u;
+    const bool with_pos = h.pos_x != nullptr;
+    uint32_t my = tid;
+    bool active = base + tid < n;
+    RayState st;
+    if (active) {
+        uint32_t i = base + tid;
+        ray_setup(st, r.ox[i], r.oy[i], r.oz[i], r.dx[i], r.dy[i], r.dz[i], r.W[0], r.W[1], r.W[2]);
+    }
+    unsigned long long c_rays = 0, c_iters = 0, c_desc = 0, c_hits = 0;
+    int32_t v[3] = {0, 0, 0};
+    for (;;) {
+        bool tame = __all(!active || ray_is_tame(st));
+        bool hit = false, fin = false;
+        if (active) {
+            if (tame)
+                fin = trace_segment<false>(w, st, seg, max_iters, v, hit);
+            else
+                fin = trace_segment<true>(w, st, seg, max_iters, v, hit);
+        }
+        if (fin) {
+            float t;
+            int4 rec;
+            if (hit)
+                ray_hit_record(w, st, v, brick_slot_at(w, v), t, rec);
+            else
+                ray_miss_record(st, t, rec);
+            L.t[my] = t;
+            L.rec[my] = rec;
+            if (with_pos) {
+#pragma unroll
+                for (int a = 0; a < 3; a++) L.pos[a][my] = hit ? st.cur[a] : 0.0f;
+            }
+            if constexpr (COUNT) {
+                c_rays++;
+                c_iters += st.iters;
+                c_desc += st.descents;
+                c_hits += hit ? 1ull : 0ull;
+            }
+            active = false;
+        }
+        // pack the running rays into slots 0 .. total-1
+        uint64_t am = __ballot(active);
+        if (lane == 0) L.wave_cnt[wave] = (uint32_t)__popcll(am);
+        __syncthreads();
+        uint32_t total = 0, before = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < 4; k++) {
+            uint32_t c = L.wave_cnt[k];
+            before += k < wave ? c : 0u;
+            total += c;
+        }
+        if (total == 0) break;
+        if (active) {
+            uint32_t rank = before + __builtin_amdgcn_mbcnt_hi((uint32_t)(am >> 32),
+                                                               __builtin_amdgcn_mbcnt_lo((uint32_t)am, 0u));
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                L.o[a][rank] = st.o[a];
+                L.d[a][rank] = st.d[a];
+                L.inv[a][rank] = st.inv[a];
+                L.side[a][rank] = st.side[a];
+            }
+            L.it[rank] = st.iters | st.descents << 16;
+            L.slot[rank] = my;
+        }
+        __syncthreads();
+        active = tid < total;
+        if (active) {
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                st.o[a] = L.o[a][tid];
+                st.d[a] = L.d[a][tid];
+                st.inv[a] = L.inv[a][tid];
+                st.side[a] = L.side[a][tid];
+                st.ts[a] = ((st.d[a] < 0.0f ? 0.0f : 1.0f) - st.o[a]) * st.inv[a];
+                st.dm[a] = st.d[a] < 0.0f ? 0 : -1;
+                asm volatile("" : "+v"(st.dm[a]));
+                st.W[a] = r.W[a];
+            }
+            float tmin = min_x86(min_x86(st.side[0], st.side[1]), st.side[2]) + 0.001f;
+#pragma unroll
+            for (int a = 0; a < 3; a++) st.cur[a] = fmaf(tmin, st.d[a], st.o[a]);
+            uint32_t itd = L.it[tid];
+            st.iters = itd & 0xFFFFu;
+            st.descents = itd >> 16;
+            my = L.slot[tid];
+        }
+    }
+    // every ray of the workgroup has finished: write the results in ray order
+    __syncthreads();
+    if (base + tid < n) {
+        uint32_t i = base + tid;
+        h.t[i] = L.t[tid];
+        h.rec[i] = L.rec[tid];
+        if (with_pos) {
+            h.pos_x[i] = L.pos[0][tid];
+            h.pos_y[i] = L.pos[1][tid];
+            h.pos_z[i] = L.pos[2][tid];
+        }
+    }
+    if constexpr (COUNT) flush_counters(ctr, c_rays, c_iters, c_desc, c_hits);
+}
+
 // ---------------------------------------------------------------- MULTIPASS schedule
 
 // A wave runs until its slowest ray, so rays with long-tailed iteration counts
@@ -419,6 +560,17 @@ int launch_cast(const vrt_world* world, const RaysIn& r, const HitsOut& h, uint3
             cast_grid_kernel<true><<<blocks, 256, 0, s>>>(w, r, h, max_iters, o.counters);
         else
             cast_grid_kernel<false><<<blocks, 256, 0, s>>>(w, r, h, max_iters, nullptr);
+        VRT_HIP(hipGetLastError());
+        return VRT_OK;
+    }
+    if (sched == VRT_SCHED_REPACK) {
+        uint32_t seg = o.pass_iters ? o.pass_iters : 16;
+        if (r.n > 0xFFFFFF00u) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_cast_ex: too many rays");
+        uint32_t blocks = (r.n + 255) / 256;
+        if (o.counters)
+            cast_repack_kernel<true><<<blocks, 256, 0, s>>>(w, r, h, max_iters, seg, o.counters);
+        else
+            cast_repack_kernel<false><<<blocks, 256, 0, s>>>(w, r, h, max_iters, seg, nullptr);
         VRT_HIP(hipGetLastError());
         return VRT_OK;
     }

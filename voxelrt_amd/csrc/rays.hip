@@ -48,29 +48,38 @@ __global__ __launch_bounds__(256) void bounce_count_kernel(const int4* __restric
 }
 
 // Exclusive scan of the block counts in one workgroup; writes the total to *count.
+// Thread t owns a run of consecutive counts; runs are scanned with wave shuffles,
+// then across the 16 waves through LDS (one barrier).
 __global__ __launch_bounds__(1024) void bounce_scan_kernel(uint32_t* __restrict__ block_counts, uint32_t nblocks,
                                                            uint32_t* __restrict__ count) {
-    __shared__ uint32_t partial[1024];
+    __shared__ uint32_t wave_total[16];
+    const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
     uint32_t per = (nblocks + 1023) / 1024;
-    uint32_t begin = threadIdx.x * per, end = min(begin + per, nblocks);
+    uint32_t begin = min(t * per, nblocks), end = min(begin + per, nblocks);
     uint32_t sum = 0;
     for (uint32_t k = begin; k < end; k++) sum += block_counts[k];
-    partial[threadIdx.x] = sum;
-    __syncthreads();
-    // Hillis-Steele inclusive scan over the 1024 partial sums
-    for (uint32_t off = 1; off < 1024; off <<= 1) {
-        uint32_t v = threadIdx.x >= off ? partial[threadIdx.x - off] : 0u;
-        __syncthreads();
-        partial[threadIdx.x] += v;
-        __syncthreads();
+    // inclusive scan of the run sums within the wave
+    uint32_t incl = sum;
+#pragma unroll
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+        uint32_t y = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += y;
     }
-    uint32_t run = threadIdx.x ? partial[threadIdx.x - 1] : 0u;
+    if (lane == 63) wave_total[wave] = incl;
+    __syncthreads();
+    uint32_t run = incl - sum, total = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < 16; w++) {
+        uint32_t c = wave_total[w];
+        run += w < wave ? c : 0u;
+        total += c;
+    }
     for (uint32_t k = begin; k < end; k++) {
         uint32_t c = block_counts[k];
         block_counts[k] = run;
         run += c;
     }
-    if (threadIdx.x == 1023) *count = partial[1023];
+    if (t == 0) *count = total;
 }
 
 __global__ __launch_bounds__(256) void bounce_write_kernel(const int4* __restrict__ rec, const float* __restrict__ px,
