@@ -357,6 +357,24 @@ def main() -> int:
     log(f"bounce generation {statistics.mean(gen_ms):.3f} ms; step {statistics.mean(step_ms):.3f} ms (events), "
         f"{ms_per_step:.3f} ms (wall); cast roofline {achieved:.0f} GB/s = {achieved / HBM_PEAK_GBS:.1%} of 8 TB/s")
 
+    # the same step fused into one kernel (vrt_cast_frame: a lane traces its pixel's primary ray,
+    # then that hit's bounce ray); reported beside the plain per-ray pipeline, checked against it
+    fp = V.HitBatch.empty(n, with_pos=False)
+    fb = V.HitBatch.empty(n, with_pos=False)
+    V.cast_frame(world, cam, W.MAX_ITERS, W.BOUNCE_SEED, schedule=1, out=(fp, fb))
+    torch.cuda.synchronize()
+    src_h = src[:nb].long()
+    frame_ok = bool(torch.equal(fp.rec, hits_p.rec) and torch.equal(fb.rec[src_h], hits_b.rec[:nb]))
+    frame_ms = _time_ms(lambda: V.cast_frame(world, cam, W.MAX_ITERS, W.BOUNCE_SEED, schedule=1, out=(fp, fb)), K)
+    fused = {
+        "value": round((n + nb) / frame_ms / 1e3, 1),
+        "unit": "Mrays/s",
+        "ms_per_step": round(frame_ms, 4),
+        "kernel": "vrt_cast_frame (grid): primary + bounce per lane",
+        "parity_with_plain": frame_ok,
+    }
+    log(f"fused frame {fused['value']} Mrays/s ({frame_ms:.3f} ms), parity with the plain pipeline {frame_ok}")
+
     cpu = None
     if not args.no_cpu:
         sys.path.insert(0, str(ROOT / "oracle"))
@@ -418,6 +436,7 @@ def main() -> int:
             "parallelism": "single GPU",
         },
         "roofline": roofline,
+        "fused_frame": fused,
         "cpu_baseline": cpu,
         "batched": batched,
         "differentiable": differentiable,
