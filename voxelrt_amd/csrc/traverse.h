@@ -73,6 +73,13 @@ __device__ __forceinline__ uint32_t bitfield_insert(uint32_t mask, uint32_t a, u
     return (mask & a) | (~mask & b);
 }
 
+// (a << sh) | b in one V_LSHL_OR_B32.
+__device__ __forceinline__ uint32_t lshl_or(uint32_t a, uint32_t sh, uint32_t b) {
+    uint32_t r;
+    asm("v_lshl_or_b32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "s"(sh), "v"(b));
+    return r;
+}
+
 // floor(x) as int32 in one V_CVT_FLR_I32_F32; x finite and |x| < 2^31.
 __device__ __forceinline__ int32_t floor_i32_in_range(float x) {
     int32_t f;
@@ -110,26 +117,28 @@ __device__ __forceinline__ bool ray_advance(const WorldView& w, RayState& r, int
     }
     r.iters++;
     uint32_t ux = (uint32_t)v[0], uy = (uint32_t)v[1], uz = (uint32_t)v[2];
-    uint32_t si = (ux >> 5) | (uz >> 5) << w.shift_xz | (uy >> 5) << (2 * w.shift_xz);
+    uint32_t si = lshl_or(uy >> 5, 2 * w.shift_xz, lshl_or(uz >> 5, w.shift_xz, ux >> 5));
     uint4 sec = w.sec[si];
     // keep the base slot in the same 16-byte load as the mask: a descent then waits on
     // one load (the occupancy mask) instead of two
     asm volatile("" : "+v"(sec.z));
     uint64_t m = (uint64_t)sec.x | (uint64_t)sec.y << 32;
     // brick within the sector: bits 3..4 of x, z, y at bits 0..1, 2..3, 4..5
-    uint32_t bi = bitfield_insert(15u, bitfield_insert(3u, __builtin_amdgcn_ubfe(ux, 3, 2), uz >> 1), uy << 1);
-    uint32_t level = 3;  // log2 of the cell edge below the mask: a brick, 8 voxels
-    uint64_t below = m << ((63u - bi) & 63u);
-    if ((int32_t)(below >> 32) < 0) {
+    // nbi = ~bi: bits 0..5 give the shift 63 - bi, and ~nbi & 0x2A the 2x2x2 group's shift
+    uint32_t nbi = ~bitfield_insert(15u, bitfield_insert(3u, __builtin_amdgcn_ubfe(ux, 3, 2), uz >> 1), uy << 1);
+    asm volatile("" : "+v"(nbi));
+    uint64_t below = m << (nbi & 63u);
+    bool descend = (int32_t)(below >> 32) < 0;
+    if (descend) {
         r.descents++;
         uint32_t slot1 = sec.z + (uint32_t)__popcll(below);
         uint32_t cell = __builtin_amdgcn_ubfe(ux, 2, 1) | __builtin_amdgcn_ubfe(uz, 2, 1) << 1 |
                         __builtin_amdgcn_ubfe(uy, 2, 1) << 2;
         m = w.occ[(size_t)(slot1 * 8u + cell) - 8u];
         // voxel within the brick's 4x4x4 cell: bits 0..1 of x, z, y
-        bi = bitfield_insert(15u, bitfield_insert(3u, ux, uz << 2), uy << 4);
-        level = 0;
-        if ((int32_t)((m << ((63u - bi) & 63u)) >> 32) < 0) {
+        nbi = ~bitfield_insert(15u, bitfield_insert(3u, ux, uz << 2), uy << 4);
+        asm volatile("" : "+v"(nbi));
+        if ((int32_t)((m << (nbi & 63u)) >> 32) < 0) {
             *hit_slot = slot1 - 1;
             hit = true;
             return false;
@@ -137,8 +146,9 @@ __device__ __forceinline__ bool ray_advance(const WorldView& w, RayState& r, int
     }
     // the 2x2x2 group holding bi: (half >> (bi & 0xA)) & 0x00330033 on the 32-bit
     // half selected by bit 5 of bi
-    uint32_t grp = (uint32_t)(m >> (bi & 0x2Au)) & 0x00330033u;
-    uint32_t s = level + (grp == 0 ? 1u : 0u) + (m == 0 ? 1u : 0u);
+    uint32_t grp = (uint32_t)(m >> (~nbi & 0x2Au)) & 0x00330033u;
+    // log2 of the cell edge below the mask (a brick: 8 voxels; a voxel: 1) plus the LOD
+    uint32_t s = (descend ? 0u : 3u) + (grp == 0 ? 1u : 0u) + (m == 0 ? 1u : 0u);
     uint32_t cm = (1u << s) - 1u;
 #pragma unroll
     for (int a = 0; a < 3; a++) {
