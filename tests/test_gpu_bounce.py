@@ -106,3 +106,29 @@ def test_device_count_bounds_the_cast(vrt, oracle_mod, cuda):
             assert int(ctr[0].item()) == k
             assert np.all(rec[k:] == -7)
             assert not np.any(np.all(rec[:k] == -7, axis=1))
+
+
+def test_bounce_compaction_of_many_blocks(vrt, cuda):
+    """More than 15872 blocks of 256 primaries (the scan's LDS staging limit): the scan takes
+    its global-memory path; the compacted bounce set equals the host twin."""
+    import voxelrt_amd as V
+    from voxelrt_amd import _lib as L
+    from voxelrt_amd import workloads as W
+
+    scene = W.C1_WORLD.build()
+    world = V.World.from_scene(scene)
+    cam = W.c1_camera(tile=(8, 8), width=2560, height=1600)   # 4 096 000 rays, 16 000 blocks
+    prim = cam.rays_device()
+    hits, br, src, nb, hb = run_pipeline(V, world, prim, cam.c.width, 5, W.MAX_ITERS)
+    n = prim.n
+    rec = np.ascontiguousarray(hits.rec.cpu().numpy())
+    pos = np.ascontiguousarray(hits.pos.cpu().numpy())
+    out = np.zeros((6, n), np.float32)
+    hsrc = np.zeros(n, np.uint32)
+    cnt = C.c_uint32()
+    ptrs = (C.c_void_p * 6)(*[out[i].ctypes.data for i in range(6)])
+    L.check(V.lib().vrt_bounce_host(n, rec.ctypes.data, pos.ctypes.data, cam.c.width, 5, C.byref(ptrs),
+                                    hsrc.ctypes.data, C.byref(cnt)))
+    assert cnt.value == nb and nb > 1_000_000
+    assert np.array_equal(src[:nb].cpu().numpy().astype(np.uint32), hsrc[:nb])
+    assert np.array_equal(out[3:, :nb].view(np.uint32), br.d[:, :nb].cpu().numpy().view(np.uint32))

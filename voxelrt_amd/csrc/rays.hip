@@ -48,17 +48,13 @@ __global__ __launch_bounds__(256) void bounce_count_kernel(const int4* __restric
 }
 
 // Exclusive scan of the block counts in one workgroup; writes the total to *count.
-// Thread t owns a run of consecutive counts; runs are scanned with wave shuffles,
-// then across the 16 waves through LDS (one barrier).
-__global__ __launch_bounds__(1024) void bounce_scan_kernel(uint32_t* __restrict__ block_counts, uint32_t nblocks,
-                                                           uint32_t* __restrict__ count) {
-    __shared__ uint32_t wave_total[16];
+// Up to kScanLds counts are staged through LDS with coalesced loads and stores;
+// thread t scans a run of consecutive counts, runs are combined with wave
+// shuffles and across the 16 waves through LDS.
+static const uint32_t kScanLds = 15872;  // 62 KiB of LDS
+
+__device__ __forceinline__ uint32_t scan_runs(uint32_t sum, uint32_t* wave_total, uint32_t& total) {
     const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
-    uint32_t per = (nblocks + 1023) / 1024;
-    uint32_t begin = min(t * per, nblocks), end = min(begin + per, nblocks);
-    uint32_t sum = 0;
-    for (uint32_t k = begin; k < end; k++) sum += block_counts[k];
-    // inclusive scan of the run sums within the wave
     uint32_t incl = sum;
 #pragma unroll
     for (uint32_t off = 1; off < 64; off <<= 1) {
@@ -67,17 +63,42 @@ __global__ __launch_bounds__(1024) void bounce_scan_kernel(uint32_t* __restrict_
     }
     if (lane == 63) wave_total[wave] = incl;
     __syncthreads();
-    uint32_t run = incl - sum, total = 0;
+    uint32_t run = incl - sum;
+    total = 0;
 #pragma unroll
     for (uint32_t w = 0; w < 16; w++) {
         uint32_t c = wave_total[w];
         run += w < wave ? c : 0u;
         total += c;
     }
+    return run;  // exclusive prefix of this thread's run
+}
+
+__global__ __launch_bounds__(1024) void bounce_scan_kernel(uint32_t* __restrict__ block_counts, uint32_t nblocks,
+                                                           uint32_t* __restrict__ count) {
+    __shared__ uint32_t wave_total[16];
+    __shared__ uint32_t stage[kScanLds];
+    const uint32_t t = threadIdx.x;
+    const bool staged = nblocks <= kScanLds;
+    uint32_t* buf = staged ? stage : block_counts;
+    if (staged) {
+        for (uint32_t k = t; k < nblocks; k += 1024) stage[k] = block_counts[k];
+        __syncthreads();
+    }
+    uint32_t per = (nblocks + 1023) / 1024;
+    uint32_t begin = min(t * per, nblocks), end = min(begin + per, nblocks);
+    uint32_t sum = 0;
+    for (uint32_t k = begin; k < end; k++) sum += buf[k];
+    uint32_t total;
+    uint32_t run = scan_runs(sum, wave_total, total);
     for (uint32_t k = begin; k < end; k++) {
-        uint32_t c = block_counts[k];
-        block_counts[k] = run;
+        uint32_t c = buf[k];
+        buf[k] = run;
         run += c;
+    }
+    if (staged) {
+        __syncthreads();
+        for (uint32_t k = t; k < nblocks; k += 1024) block_counts[k] = stage[k];
     }
     if (t == 0) *count = total;
 }
