@@ -342,6 +342,16 @@ def main() -> int:
         "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": None,
     }
+    # achievable HBM bandwidth on this device, measured in the same run: a 1 GiB device-to-device
+    # copy (reads and writes 1 GiB each)
+    ca = torch.empty(1 << 28, dtype=torch.float32, device=dev)
+    cb = torch.empty_like(ca)
+    ca.fill_(1.0)
+    copy_ms = _time_ms(lambda: cb.copy_(ca), 10)
+    achievable = 2 * ca.numel() * 4 / (copy_ms / 1e3) / 1e9
+    del ca, cb
+    roofline["achievable"] = round(achievable, 1)
+    roofline["frac_of_achievable"] = round(achieved / achievable, 4)
     traffic_file = ROOT / "profiles" / "traffic_cast_c2.json"
     if traffic_file.exists():
         try:
@@ -355,7 +365,8 @@ def main() -> int:
     log(f"bounce  {nb} rays: {statistics.mean(bounce_ms):.3f} ms ({nb / statistics.mean(bounce_ms) / 1e3:.0f} Mrays/s), "
         f"iters/ray {cb_h[1] / max(nb, 1):.2f}, descents/ray {cb_h[2] / max(nb, 1):.2f}, hit {cb_h[3] / max(nb, 1):.3f}")
     log(f"bounce generation {statistics.mean(gen_ms):.3f} ms; step {statistics.mean(step_ms):.3f} ms (events), "
-        f"{ms_per_step:.3f} ms (wall); cast roofline {achieved:.0f} GB/s = {achieved / HBM_PEAK_GBS:.1%} of 8 TB/s")
+        f"{ms_per_step:.3f} ms (wall); cast roofline {achieved:.0f} GB/s = {achieved / HBM_PEAK_GBS:.1%} of 8 TB/s, "
+        f"{achieved / achievable:.1%} of the {achievable:.0f} GB/s a 1 GiB device copy reaches")
 
     # the same step fused into one kernel (vrt_cast_frame: a lane traces its pixel's primary ray,
     # then that hit's bounce ray); reported beside the plain per-ray pipeline, checked against it
