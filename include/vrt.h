@@ -111,7 +111,9 @@ int vrt_world_set_palette(vrt_world* world, const uint64_t* palette, void* strea
 typedef struct vrt_sector_record {
     uint64_t mask;       /* bit b: brick b (x | z << 2 | y << 4 in the sector) is allocated */
     uint32_t base_slot;  /* slot of the first allocated brick */
-    uint32_t reserved;
+    uint32_t level;      /* 3 + (mask == 0): log2 of the step over the sector's empty bricks before
+                            the 2x2x2-group LOD (GetStepPos, CpuRenderer.cpp:158-160), kept here so
+                            the traversal does not test the mask for 0 */
 } vrt_sector_record;
 
 /* Device views of the world buffers (read-only use by callers). */

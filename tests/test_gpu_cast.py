@@ -83,6 +83,24 @@ def test_random_world_parity(vrt, oracle_mod, cuda, seed, size, inside, max_iter
         assert np.array_equal(t.view(np.uint32), ref["t"].view(np.uint32))
 
 
+@pytest.mark.parametrize("size,ysize,seed", [(128, 64, 21), (64, 128, 22), (128, 32, 23)])
+def test_non_cubic_world_parity(vrt, oracle_mod, cuda, size, ysize, seed):
+    """Worlds whose y extent differs from x/z take the two-compare bounds test; rays start
+    inside and around the box so that every face of it ends some of them."""
+    vox = blob_world(size, seed=seed, density=0.01, ysize=ysize)
+    world, ow = pair_from_dense(vrt, oracle_mod, vox)
+    info = world.info()
+    assert (info.size_xz, info.size_y) == (size, ysize)
+    for inside in (True, False):
+        o, d = random_rays(20000, size, seed=seed + 100, ysize=ysize, inside=inside)
+        ref = ow.cast(o, d, (0, 0, 0), 128)
+        for sched, chunk in SCHEDULES:
+            t, rec, _ = gpu_cast(world, o, d, (0, 0, 0), 128, sched, chunk)
+            assert_hits_equal(t, rec, ref["t"], ref["rec"], f"{size}x{ysize} inside={inside} sched={sched}")
+            assert np.array_equal(t.view(np.uint32), ref["t"].view(np.uint32))
+        assert ((ref["rec"][:, 3] & 0xFF) != 0).mean() > 0.05
+
+
 def test_special_rays(vrt, oracle_mod, cuda):
     """Exact zeros (inf/NaN side distances), denormal and huge components, far origins (floor2i
     out of range), origins on voxel faces, empty and solid worlds."""

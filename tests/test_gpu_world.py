@@ -43,6 +43,29 @@ def cast_np(world, o, d, W=(0, 0, 0), max_iters=128):
     return h.t.cpu().numpy(), h.rec.cpu().numpy()
 
 
+class _DeviceWords:
+    """A device buffer of u32 words, for torch.as_tensor."""
+
+    def __init__(self, ptr: int, shape: tuple):
+        self.__cuda_array_interface__ = {"shape": shape, "typestr": "<u4", "data": (ptr, True), "version": 3}
+
+
+def sector_records(world) -> np.ndarray:
+    """The world's device sector records as [num_sectors, 4] u32: mask lo, mask hi, base, level."""
+    n = world.info().num_sectors
+    torch.cuda.synchronize()
+    return torch.as_tensor(_DeviceWords(world.buffers().sectors, (n, 4)), device="cuda").cpu().numpy()
+
+
+def check_sector_levels(world, masks_by_sector: dict):
+    rec = sector_records(world)
+    mask = rec[:, 0].astype(np.uint64) | rec[:, 1].astype(np.uint64) << np.uint64(32)
+    for si, m in masks_by_sector.items():
+        assert int(mask[si]) == int(m), si
+    # level = log2 of the empty-brick step before the group LOD: 3, or 4 for an empty sector
+    assert np.array_equal(rec[:, 3], np.where(mask == 0, 4, 3).astype(np.uint32))
+
+
 def test_incremental_sync_equals_fresh_world(vrt, oracle_mod, cuda):
     import voxelrt_amd as V
     from test_oracle import dense_world
@@ -51,7 +74,9 @@ def test_incremental_sync_equals_fresh_world(vrt, oracle_mod, cuda):
     vox = blob_world(128, seed=21, density=0.01)
     world = V.World(7, 7)
     nsec = 4 ** 3
+    check_sector_levels(world, {si: 0 for si in range(nsec)})
     world.sync(*sector_stream_of(vox, list(range(nsec))))
+    check_sector_levels(world, dict(zip(range(nsec), sector_stream_of(vox, list(range(nsec)))[1])))
     o, d = random_rays(20000, 128, seed=5)
     for step in range(4):
         # edit: clear a box, paint another, empty a whole sector
@@ -77,6 +102,9 @@ def test_incremental_sync_equals_fresh_world(vrt, oracle_mod, cuda):
         fresh = V.World.from_scene(V.Scene.from_dense(vox))
         t2, rec2 = cast_np(fresh, o, d)
         assert np.array_equal(rec, rec2)
+        masks = sector_stream_of(vox, list(range(nsec)))[1]
+        assert (masks == 0).any()  # the emptied sector
+        check_sector_levels(world, dict(zip(range(nsec), masks)))
         info = world.info()
         assert info.num_bricks == sum(bin(int(m)).count("1") for m in sector_stream_of(vox, list(range(nsec)))[1])
 

@@ -121,7 +121,7 @@ __device__ __forceinline__ void trace_block(const WorldView& w, const BatchArgs&
     for (int k = 0; k < RPL; k++) hit[k] = false;
     if constexpr (RPL == 1) {
         if (!done[0])
-            hit[0] = wave_tame ? ray_trace<false>(w, st[0], a.max_iters, v[0]) : ray_trace<true>(w, st[0], a.max_iters, v[0]);
+            hit[0] = ray_trace_wave(w, st[0], a.max_iters, v[0], wave_tame);
     } else {
         // the lane's rays step in lockstep, so one counter caps them all
         uint32_t unused;

@@ -54,10 +54,10 @@ __device__ __forceinline__ void store_result(const HitsOut& h, uint32_t i, const
 }
 
 // Traces one ray to completion.
-template <bool EXACT>
-__device__ __forceinline__ void trace_full(const WorldView& w, RayState& st, uint32_t max_iters, float& t, int4& rec) {
+__device__ __forceinline__ void trace_full(const WorldView& w, RayState& st, uint32_t max_iters, bool wave_tame,
+                                           float& t, int4& rec) {
     int32_t v[3];
-    if (ray_trace<EXACT>(w, st, max_iters, v))
+    if (ray_trace_wave(w, st, max_iters, v, wave_tame))
         ray_hit_record(w, st, v, brick_slot_at(w, v), t, rec);
     else
         ray_miss_record(st, t, rec);
@@ -76,10 +76,7 @@ __global__ __launch_bounds__(256) void cast_grid_kernel(WorldView w, RaysIn r, H
     if (valid) {
         ray_setup(st, r.ox[i], r.oy[i], r.oz[i], r.dx[i], r.dy[i], r.dz[i], r.W[0], r.W[1], r.W[2]);
         // the fast loop when every ray of the wave is tame, else the exact one
-        if (__all(ray_is_tame(st)))
-            trace_full<false>(w, st, max_iters, t, rec);
-        else
-            trace_full<true>(w, st, max_iters, t, rec);
+        trace_full(w, st, max_iters, __all(ray_is_tame(st)), t, rec);
         store_result(h, i, st, t, rec);
     }
     if constexpr (COUNT) {

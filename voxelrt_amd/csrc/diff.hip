@@ -38,7 +38,7 @@ __global__ __launch_bounds__(256) void render_fwd_kernel(WorldView w, vrt_camera
     }
     bool wave_tame = __all(tame);
     if (valid) {
-        bool hit = wave_tame ? ray_trace<false>(w, st, max_iters, v) : ray_trace<true>(w, st, max_iters, v);
+        bool hit = ray_trace_wave(w, st, max_iters, v, wave_tame);
         float wgt = 0.0f;
         uint32_t gv = 0xFFFFFFFFu;
         if (hit) {

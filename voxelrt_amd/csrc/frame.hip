@@ -99,7 +99,7 @@ __device__ __forceinline__ bool trace_wave(const WorldView& w, RayState& st, boo
                                            int32_t v[3]) {
     bool tame = __all(!active || ray_is_tame(st));
     if (!active) return false;
-    return tame ? ray_trace<false>(w, st, max_iters, v) : ray_trace<true>(w, st, max_iters, v);
+    return ray_trace_wave(w, st, max_iters, v, tame);
 }
 
 __device__ __forceinline__ void finish_trace(const WorldView& w, const RayState& st, bool hit, const int32_t v[3],

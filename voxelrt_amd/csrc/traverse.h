@@ -80,6 +80,25 @@ __device__ __forceinline__ uint32_t lshl_or(uint32_t a, uint32_t sh, uint32_t b)
     return r;
 }
 
+// (M & a) | (~M & b) in one V_BFI_B32 for a constant field mask M (written out,
+// because the compiler splits a constant-mask insert into ANDs once it knows which
+// result bits are used).
+template <uint32_t M>
+__device__ __forceinline__ uint32_t insert_bits(uint32_t a, uint32_t b) {
+    uint32_t r;
+    asm("v_bfi_b32 %0, %1, %2, %3" : "=v"(r) : "i"(M), "v"(a), "v"(b));
+    return r;
+}
+
+// ~((M & a) | (~M & b)) in one V_BITOP3_B32: the truth table is indexed by
+// (M bit) * 4 + (a bit) * 2 + (b bit), so 0x35 is "not (M ? a : b)".
+template <uint32_t M>
+__device__ __forceinline__ uint32_t insert_bits_not(uint32_t a, uint32_t b) {
+    uint32_t r;
+    asm("v_bitop3_b32 %0, %1, %2, %3 bitop3:0x35" : "=v"(r) : "i"(M), "v"(a), "v"(b));
+    return r;
+}
+
 // floor(x) as int32 in one V_CVT_FLR_I32_F32; x finite and |x| < 2^31.
 __device__ __forceinline__ int32_t floor_i32_in_range(float x) {
     int32_t f;
@@ -102,7 +121,7 @@ __device__ __forceinline__ int32_t floor_i32_in_range(float x) {
 //     reference's LOD of +1 / +2 (a 4x4x4 mask that is 0 has every group empty);
 //   alignment: (v & ~cm) | (dm & cm), one bitfield insert per axis.
 // Occupancy indices slot * 8 + cell are 32-bit: worlds hold at most 2^28 slots.
-template <bool EXACT>
+template <bool EXACT, bool CUBIC = false>
 __device__ __forceinline__ bool ray_advance(const WorldView& w, RayState& r, int32_t v[3], bool& hit,
                                             uint32_t* hit_slot) {
 #pragma unroll
@@ -110,45 +129,54 @@ __device__ __forceinline__ bool ray_advance(const WorldView& w, RayState& r, int
         int32_t f = EXACT ? floor2i(r.cur[a]) : floor_i32_in_range(r.cur[a]);
         v[a] = (int32_t)((uint32_t)r.W[a] + (uint32_t)f);
     }
-    // GetInboundMask (:114-117)
-    if (!((uint32_t)(v[0] | v[2]) < w.size_xz && (uint32_t)v[1] < w.size_y)) {
+    // GetInboundMask (:114-117); one compare when the world is a cube
+    bool inb = CUBIC ? (uint32_t)(v[0] | v[1] | v[2]) < w.size_xz
+                     : ((uint32_t)(v[0] | v[2]) < w.size_xz && (uint32_t)v[1] < w.size_y);
+    if (!inb) {
         hit = false;
         return false;
     }
     r.iters++;
     uint32_t ux = (uint32_t)v[0], uy = (uint32_t)v[1], uz = (uint32_t)v[2];
     uint32_t si = lshl_or(uy >> 5, 2 * w.shift_xz, lshl_or(uz >> 5, w.shift_xz, ux >> 5));
-    uint4 sec = w.sec[si];
-    // keep the base slot in the same 16-byte load as the mask: a descent then waits on
-    // one load (the occupancy mask) instead of two
-    asm volatile("" : "+v"(sec.z));
+    // one 16-byte load for the whole record, held as one register quad: a descent then
+    // waits on one load (the occupancy mask) instead of two
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    u32x4 sec = *(const u32x4*)&w.sec[si];
+    asm volatile("" : "+v"(sec));
     uint64_t m = (uint64_t)sec.x | (uint64_t)sec.y << 32;
     // brick within the sector: bits 3..4 of x, z, y at bits 0..1, 2..3, 4..5
     // nbi = ~bi: bits 0..5 give the shift 63 - bi, and ~nbi & 0x2A the 2x2x2 group's shift
-    uint32_t nbi = ~bitfield_insert(15u, bitfield_insert(3u, __builtin_amdgcn_ubfe(ux, 3, 2), uz >> 1), uy << 1);
-    asm volatile("" : "+v"(nbi));
+    uint32_t nbi = insert_bits_not<0x30u>(uy << 1, insert_bits<0x0Cu>(uz >> 1, ux >> 3));
     uint64_t below = m << (nbi & 63u);
     bool descend = (int32_t)(below >> 32) < 0;
+    // log2 of the cell edge below the mask (a brick: 8 voxels; a voxel: 1) plus the
+    // +1 LOD of an empty mask: the sector record carries it for the sector's mask
+    uint32_t lvl = sec.w;
+    bool solid = false;
+    uint32_t slot1 = 0;
     if (descend) {
         r.descents++;
-        uint32_t slot1 = sec.z + (uint32_t)__popcll(below);
+        slot1 = sec.z + (uint32_t)__popcll(below);
         uint32_t cell = __builtin_amdgcn_ubfe(ux, 2, 1) | __builtin_amdgcn_ubfe(uz, 2, 1) << 1 |
                         __builtin_amdgcn_ubfe(uy, 2, 1) << 2;
         m = w.occ[(size_t)(slot1 * 8u + cell) - 8u];
         // voxel within the brick's 4x4x4 cell: bits 0..1 of x, z, y
-        nbi = ~bitfield_insert(15u, bitfield_insert(3u, ux, uz << 2), uy << 4);
-        asm volatile("" : "+v"(nbi));
-        if ((int32_t)((m << (nbi & 63u)) >> 32) < 0) {
-            *hit_slot = slot1 - 1;
-            hit = true;
-            return false;
-        }
+        nbi = insert_bits_not<0x30u>(uy << 4, insert_bits<0x0Cu>(uz << 2, ux));
+        solid = (int32_t)((m << (nbi & 63u)) >> 32) < 0;
+        lvl = m == 0 ? 1u : 0u;
+    }
+    // the hit test after the join: its complement is then a scalar mask operation
+    // rather than a second vector compare of the sector mask
+    if (solid) {
+        *hit_slot = slot1 - 1;
+        hit = true;
+        return false;
     }
     // the 2x2x2 group holding bi: (half >> (bi & 0xA)) & 0x00330033 on the 32-bit
     // half selected by bit 5 of bi
     uint32_t grp = (uint32_t)(m >> (~nbi & 0x2Au)) & 0x00330033u;
-    // log2 of the cell edge below the mask (a brick: 8 voxels; a voxel: 1) plus the LOD
-    uint32_t s = (descend ? 0u : 3u) + (grp == 0 ? 1u : 0u) + (m == 0 ? 1u : 0u);
+    uint32_t s = lvl + (grp == 0 ? 1u : 0u);
     uint32_t cm = (1u << s) - 1u;
 #pragma unroll
     for (int a = 0; a < 3; a++) {
@@ -192,13 +220,22 @@ __device__ __forceinline__ uint32_t brick_slot_at(const WorldView& w, const int3
 // RayCast's loop (CpuRenderer.cpp:185-202): steps until a hit, a miss or
 // max_iters steps; true on a hit, with v the hit voxel (its slot is
 // brick_slot_at(w, v)).
-template <bool EXACT>
+template <bool EXACT, bool CUBIC = false>
 __device__ __forceinline__ bool ray_trace(const WorldView& w, RayState& r, uint32_t max_iters, int32_t v[3]) {
     bool hit = false;
     uint32_t unused;
     for (uint32_t it = 0; it < max_iters; it++)
-        if (!ray_advance<EXACT>(w, r, v, hit, &unused)) break;
+        if (!ray_advance<EXACT, CUBIC>(w, r, v, hit, &unused)) break;
     return hit;
+}
+
+// ray_trace with the step picked per wave: the fast step when every ray of the
+// wave is tame (wave_tame), with the single-compare bounds test in cubic worlds.
+__device__ __forceinline__ bool ray_trace_wave(const WorldView& w, RayState& r, uint32_t max_iters, int32_t v[3],
+                                               bool wave_tame) {
+    if (!wave_tame) return ray_trace<true>(w, r, max_iters, v);
+    if (w.size_y == w.size_xz) return ray_trace<false, true>(w, r, max_iters, v);
+    return ray_trace<false>(w, r, max_iters, v);
 }
 
 // Hit record fields (vrt_hit in vrt.h) of a finished ray.

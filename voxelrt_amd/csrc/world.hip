@@ -263,7 +263,10 @@ int vrt_world_create(const vrt_world_desc* desc, void* stream, vrt_world** out) 
         check(hipMalloc(&w->d_groups, w->h_groups.size() * 8), "hipMalloc(sector groups)");
         check(hipMalloc(&w->d_lut, 512 * 8), "hipMalloc(ray-cell table)");
         if (rc == VRT_OK) {
-            check(hipMemsetAsync(w->d_sec, 0, (size_t)w->nsec * 16, s), "hipMemsetAsync(sector records)");
+            std::vector<uint4> empty(w->nsec, sector_record(0, 0));
+            check(hipMemcpyAsync(w->d_sec, empty.data(), (size_t)w->nsec * 16, hipMemcpyHostToDevice, s),
+                  "hipMemcpyAsync(sector records)");
+            check(hipStreamSynchronize(s), "hipStreamSynchronize(sector records)");
             check(hipMemsetAsync(w->d_palette, 0, 256 * 8, s), "hipMemsetAsync(palette)");
             check(hipMemsetAsync(w->d_groups, 0, w->h_groups.size() * 8, s), "hipMemsetAsync(sector groups)");
             check(hipMemcpyAsync(w->d_lut, lut.data(), 512 * 8, hipMemcpyHostToDevice, s), "hipMemcpyAsync(ray-cell table)");
@@ -407,7 +410,7 @@ int vrt_world_sync(vrt_world* w, uint32_t nsec, const uint32_t* sector_idx, cons
         }
         std::vector<uint4> recs(w->nsec);
         for (uint32_t i = 0; i < w->nsec; i++)
-            recs[i] = make_uint4((uint32_t)w->h_mask[i], (uint32_t)(w->h_mask[i] >> 32), w->h_base[i], 0u);
+            recs[i] = sector_record(w->h_mask[i], w->h_base[i]);
         VRT_HIP(hipMemcpyAsync(w->d_sec, recs.data(), (size_t)w->nsec * 16, hipMemcpyHostToDevice, s));
         VRT_HIP(hipMemcpyAsync(w->d_groups, w->h_groups.data(), w->h_groups.size() * 8, hipMemcpyHostToDevice, s));
         hipError_t e = hipStreamSynchronize(s);
