@@ -111,9 +111,10 @@ int vrt_world_set_palette(vrt_world* world, const uint64_t* palette, void* strea
 typedef struct vrt_sector_record {
     uint64_t mask;       /* bit b: brick b (x | z << 2 | y << 4 in the sector) is allocated */
     uint32_t base_slot;  /* slot of the first allocated brick */
-    uint32_t level;      /* 3 + (mask == 0): log2 of the step over the sector's empty bricks before
-                            the 2x2x2-group LOD (GetStepPos, CpuRenderer.cpp:158-160), kept here so
-                            the traversal does not test the mask for 0 */
+    uint32_t empty_group_cell; /* 15, or 31 when mask is 0: the cell mask (edge - 1) of a step that
+                                  stays at the sector level where the 2x2x2 brick group is empty,
+                                  i.e. the LOD of GetStepPos (CpuRenderer.cpp:158-160) precomputed
+                                  per sector; the step takes half when the group holds a brick */
 } vrt_sector_record;
 
 /* Device views of the world buffers (read-only use by callers). */

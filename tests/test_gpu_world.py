@@ -47,11 +47,11 @@ class _DeviceWords:
     """A device buffer of u32 words, for torch.as_tensor."""
 
     def __init__(self, ptr: int, shape: tuple):
-        self.__cuda_array_interface__ = {"shape": shape, "typestr": "<u4", "data": (ptr, True), "version": 3}
+        self.__cuda_array_interface__ = {"shape": shape, "typestr": "<u4", "data": (ptr, False), "version": 3}
 
 
 def sector_records(world) -> np.ndarray:
-    """The world's device sector records as [num_sectors, 4] u32: mask lo, mask hi, base, level."""
+    """The world's device sector records as [num_sectors, 4] u32: mask lo, mask hi, base, group cell mask."""
     n = world.info().num_sectors
     torch.cuda.synchronize()
     return torch.as_tensor(_DeviceWords(world.buffers().sectors, (n, 4)), device="cuda").cpu().numpy()
@@ -62,8 +62,8 @@ def check_sector_levels(world, masks_by_sector: dict):
     mask = rec[:, 0].astype(np.uint64) | rec[:, 1].astype(np.uint64) << np.uint64(32)
     for si, m in masks_by_sector.items():
         assert int(mask[si]) == int(m), si
-    # level = log2 of the empty-brick step before the group LOD: 3, or 4 for an empty sector
-    assert np.array_equal(rec[:, 3], np.where(mask == 0, 4, 3).astype(np.uint32))
+    # cell mask of a step over an empty brick group: 16 voxels, the whole sector when empty
+    assert np.array_equal(rec[:, 3], np.where(mask == 0, 31, 15).astype(np.uint32))
 
 
 def test_incremental_sync_equals_fresh_world(vrt, oracle_mod, cuda):

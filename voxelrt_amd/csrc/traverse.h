@@ -99,6 +99,13 @@ __device__ __forceinline__ uint32_t insert_bits_not(uint32_t a, uint32_t b) {
     return r;
 }
 
+// min(x, 1) in one V_MIN_U32 (the compiler would compare and select).
+__device__ __forceinline__ uint32_t min_1(uint32_t x) {
+    uint32_t r;
+    asm("v_min_u32 %0, %1, 1" : "=v"(r) : "v"(x));
+    return r;
+}
+
 // floor(x) as int32 in one V_CVT_FLR_I32_F32; x finite and |x| < 2^31.
 __device__ __forceinline__ int32_t floor_i32_in_range(float x) {
     int32_t f;
@@ -118,7 +125,9 @@ __device__ __forceinline__ int32_t floor_i32_in_range(float x) {
 //     64-bit shifts read only those bits (the & 63 costs nothing), and the
 //     fields are merged with bitfield inserts;
 //   cell mask: cm = 2^s - 1 with s = level + (group empty) + (mask empty), the
-//     reference's LOD of +1 / +2 (a 4x4x4 mask that is 0 has every group empty);
+//     reference's LOD of +1 / +2 (a 4x4x4 mask that is 0 has every group empty).
+//     With ce = 2^(level + 1 + (mask empty)) - 1, the mask of an empty group, this
+//     is cm = ce >> min(group bits, 1); the sector record holds ce for its mask;
 //   alignment: (v & ~cm) | (dm & cm), one bitfield insert per axis.
 // Occupancy indices slot * 8 + cell are 32-bit: worlds hold at most 2^28 slots.
 template <bool EXACT, bool CUBIC = false>
@@ -150,9 +159,9 @@ __device__ __forceinline__ bool ray_advance(const WorldView& w, RayState& r, int
     uint32_t nbi = insert_bits_not<0x30u>(uy << 1, insert_bits<0x0Cu>(uz >> 1, ux >> 3));
     uint64_t below = m << (nbi & 63u);
     bool descend = (int32_t)(below >> 32) < 0;
-    // log2 of the cell edge below the mask (a brick: 8 voxels; a voxel: 1) plus the
-    // +1 LOD of an empty mask: the sector record carries it for the sector's mask
-    uint32_t lvl = sec.w;
+    // the cell mask of a step whose 2x2x2 group is empty: 15 below a sector mask, 31
+    // when that mask is 0 (from the record); 1 below a voxel mask, 3 when it is 0
+    uint32_t ce = sec.w;
     bool solid = false;
     uint32_t slot1 = 0;
     if (descend) {
@@ -164,7 +173,7 @@ __device__ __forceinline__ bool ray_advance(const WorldView& w, RayState& r, int
         // voxel within the brick's 4x4x4 cell: bits 0..1 of x, z, y
         nbi = insert_bits_not<0x30u>(uy << 4, insert_bits<0x0Cu>(uz << 2, ux));
         solid = (int32_t)((m << (nbi & 63u)) >> 32) < 0;
-        lvl = m == 0 ? 1u : 0u;
+        ce = m == 0 ? 3u : 1u;
     }
     // the hit test after the join: its complement is then a scalar mask operation
     // rather than a second vector compare of the sector mask
@@ -176,8 +185,7 @@ __device__ __forceinline__ bool ray_advance(const WorldView& w, RayState& r, int
     // the 2x2x2 group holding bi: (half >> (bi & 0xA)) & 0x00330033 on the 32-bit
     // half selected by bit 5 of bi
     uint32_t grp = (uint32_t)(m >> (~nbi & 0x2Au)) & 0x00330033u;
-    uint32_t s = lvl + (grp == 0 ? 1u : 0u);
-    uint32_t cm = (1u << s) - 1u;
+    uint32_t cm = ce >> min_1(grp);
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         // d < 0: v & ~cm (near corner); d >= 0: v | cm (far corner)

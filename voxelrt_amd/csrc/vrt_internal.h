@@ -77,11 +77,11 @@ private:
 // ---------------------------------------------------------------- device world view
 
 // Sector record in HBM (vrt_sector_record in vrt.h): allocation mask (lo, hi), base
-// slot, and level = log2 of the cell a ray steps over when the sector's mask does not
-// send it into a brick, before the 2x2x2-group LOD: 3 (one brick) + 1 when the mask
-// is 0, the reference's +2 of an empty mask being this +1 and the empty group's +1.
+// slot, and the cell mask of a step at the sector level whose 2x2x2 group of bricks
+// is empty: 15 (a 16-voxel cell), or 31 (the whole sector) when the mask is 0, the
+// reference's +2 LOD of an empty mask; a step whose group holds a brick uses half.
 inline uint4 sector_record(uint64_t mask, uint32_t base) {
-    return make_uint4((uint32_t)mask, (uint32_t)(mask >> 32), base, mask ? 3u : 4u);
+    return make_uint4((uint32_t)mask, (uint32_t)(mask >> 32), base, mask ? 15u : 31u);
 }
 
 struct WorldView {
