@@ -178,15 +178,19 @@ typedef enum vrt_cast_schedule {
     VRT_SCHED_PERSISTENT = 2,  /* persistent waves pull ray chunks from a queue and refill idle lanes */
     VRT_SCHED_MULTIPASS = 3,   /* passes of pass_iters lookups; rays still running are packed into
                                   dense waves again for the next pass (long-tailed ray sets) */
-    VRT_SCHED_REPACK = 4       /* workgroups of 256 rays; every pass_iters lookups the rays still running
+    VRT_SCHED_REPACK = 4,      /* workgroups of 256 rays; every pass_iters lookups the rays still running
                                   are packed into the workgroup's first waves through LDS, so waves stay
                                   dense (long-tailed ray sets such as diffuse bounces) */
+    VRT_SCHED_LANES = 5        /* each lane traces `chunk` rays (1..8, 0 = 4) one after another, the
+                                  waiting ones staged in LDS, so a lane whose ray ends starts the next
+                                  (long-tailed ray sets) */
 } vrt_cast_schedule;
 
 typedef struct vrt_cast_opts {
     uint32_t schedule;          /* vrt_cast_schedule */
     uint32_t refill;            /* persistent: free lanes that trigger a retire/refill, 1..64 (0 = 16) */
-    uint32_t chunk;             /* rays per queue grab, multiple of 64 (persistent only, 0 = default) */
+    uint32_t chunk;             /* persistent: rays per queue grab, multiple of 64 (0 = default);
+                                   lanes: rays per lane, 1..8 (0 = 4) */
     uint32_t waves_per_cu;      /* persistent occupancy target (0 = default) */
     vrt_cast_counters* counters;/* optional device pointer, accumulated (not cleared) */
     const uint32_t* count;      /* optional device pointer: rays to trace, read when the kernel

@@ -34,7 +34,7 @@ def test_primary_and_bounce_parity(vrt, oracle_mod, cuda, config):
     ow = oracle_mod.OracleWorld.from_stream(scene.stream())
     prim = cam.rays_device()
     hits, br, src, nb, hb = run_pipeline(V, world, prim, cam.c.width, W.BOUNCE_SEED, W.MAX_ITERS,
-                                         schedule=3 if config == "c1" else 4)
+                                         schedule=3 if config == "c1" else 5)
     po, pd = prim.o.cpu().numpy(), prim.d.cpu().numpy()
     ref = ow.cast_simd(po, pd, prim.world_origin, W.MAX_ITERS)
     ref_s = ow.cast(po, pd, prim.world_origin, W.MAX_ITERS, with_pos=True, begin=0, end=min(po.shape[1], 200000))
@@ -94,7 +94,7 @@ def test_device_count_bounds_the_cast(vrt, oracle_mod, cuda):
     scene, cam = W.C1_WORLD.build(), W.c1_camera()
     world = V.World.from_scene(scene)
     prim = cam.rays_device()
-    for sched in (1, 2, 3, 4):
+    for sched in (1, 2, 3, 4, 5):
         for k in (0, 1, 777, prim.n):
             cnt = torch.tensor([k], dtype=torch.int32, device="cuda")
             h = V.HitBatch.empty(prim.n)

@@ -15,8 +15,8 @@ pytestmark = pytest.mark.gpu
 GOLDEN = Path(__file__).resolve().parent / "golden"
 # (schedule, chunk or lookups per pass): grid; persistent with 64/256/1024-ray chunks;
 # multipass with the default 16 lookups x 5 passes and with 4 lookups x 8 passes;
-# repack every 16 (default) and every 3 lookups
-SCHEDULES = [(1, 0), (2, 64), (2, 256), (2, 1024), (3, 0), (3, 4), (4, 0), (4, 3)]
+# repack every 16 (default) and every 3 lookups; lanes of 4 (default), 1, 3 and 8 rays
+SCHEDULES = [(1, 0), (2, 64), (2, 256), (2, 1024), (3, 0), (3, 4), (4, 0), (4, 3), (5, 0), (5, 1), (5, 3), (5, 8)]
 
 
 def gpu_cast(world, o, d, W, max_iters=128, schedule=1, chunk=0, with_pos=False, counters=None):
@@ -28,6 +28,8 @@ def gpu_cast(world, o, d, W, max_iters=128, schedule=1, chunk=0, with_pos=False,
                        counters=counters)
     elif schedule == 4:
         h = world.cast(rb, max_iters, with_pos=with_pos, schedule=4, pass_iters=chunk, counters=counters)
+    elif schedule == 5:
+        h = world.cast(rb, max_iters, with_pos=with_pos, schedule=5, chunk=chunk, counters=counters)
     else:
         h = world.cast(rb, max_iters, with_pos=with_pos, schedule=schedule, chunk=chunk, counters=counters)
     torch.cuda.synchronize()
@@ -99,6 +101,22 @@ def test_non_cubic_world_parity(vrt, oracle_mod, cuda, size, ysize, seed):
             assert_hits_equal(t, rec, ref["t"], ref["rec"], f"{size}x{ysize} inside={inside} sched={sched}")
             assert np.array_equal(t.view(np.uint32), ref["t"].view(np.uint32))
         assert ((ref["rec"][:, 3] & 0xFF) != 0).mean() > 0.05
+
+
+@pytest.mark.parametrize("max_iters", [1, 2, 7])
+def test_tiny_iteration_caps(vrt, oracle_mod, cuda, max_iters):
+    """Caps of a few lookups, every schedule (a cap of 0 is rejected: the hit record counts
+    lookups in 16 bits and a cast takes 1..65535)."""
+    vox = blob_world(64, seed=31, density=0.05)
+    world, ow = pair_from_dense(vrt, oracle_mod, vox)
+    o, d = random_rays(5000, 64, seed=32)
+    ref = ow.cast(o, d, (0, 0, 0), max_iters)
+    for sched, chunk in SCHEDULES:
+        t, rec, _ = gpu_cast(world, o, d, (0, 0, 0), max_iters, sched, chunk)
+        assert_hits_equal(t, rec, ref["t"], ref["rec"], f"cap={max_iters} sched={sched} chunk={chunk}")
+    if max_iters == 1:
+        with pytest.raises(vrt.VrtError, match="max_iters"):
+            gpu_cast(world, o, d, (0, 0, 0), 0, 5, 0)
 
 
 def test_special_rays(vrt, oracle_mod, cuda):

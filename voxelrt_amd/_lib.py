@@ -21,6 +21,8 @@ VRT_SCHED_AUTO = 0
 VRT_SCHED_GRID = 1
 VRT_SCHED_PERSISTENT = 2
 VRT_SCHED_MULTIPASS = 3
+VRT_SCHED_REPACK = 4
+VRT_SCHED_LANES = 5
 
 VRT_SEG_CAMERA = 0
 VRT_SEG_RAYS = 1

@@ -255,13 +255,14 @@ static int launch_plan(vrt_batch_plan* p, vrt_cast_counters* ctr, hipStream_t s)
     WorldView w = p->world->view();
     const uint32_t rpl = p->tuning.rays_per_lane;
     if (p->tuning.chunk == 0) {
-        uint32_t grid = (p->args.nblocks + 3) / 4;
+        // one wave (one ray block) per workgroup, as the plain cast's grid
+        uint32_t grid = p->args.nblocks;
         if (rpl == 1) {
-            if (ctr) batch_grid_kernel<1, true><<<grid, 256, 0, s>>>(w, p->args, ctr);
-            else batch_grid_kernel<1, false><<<grid, 256, 0, s>>>(w, p->args, nullptr);
+            if (ctr) batch_grid_kernel<1, true><<<grid, 64, 0, s>>>(w, p->args, ctr);
+            else batch_grid_kernel<1, false><<<grid, 64, 0, s>>>(w, p->args, nullptr);
         } else {
-            if (ctr) batch_grid_kernel<2, true><<<grid, 256, 0, s>>>(w, p->args, ctr);
-            else batch_grid_kernel<2, false><<<grid, 256, 0, s>>>(w, p->args, nullptr);
+            if (ctr) batch_grid_kernel<2, true><<<grid, 64, 0, s>>>(w, p->args, ctr);
+            else batch_grid_kernel<2, false><<<grid, 64, 0, s>>>(w, p->args, nullptr);
         }
     } else {
         uint32_t claim = std::max<uint32_t>(1, p->tuning.chunk / (64 * rpl));

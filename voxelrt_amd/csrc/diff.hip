@@ -143,7 +143,7 @@ int vrt_render_fwd(const vrt_world* world, const vrt_camera* cameras, uint32_t n
             if (!c.width || !c.height || !c.tile_w || !c.tile_h || c.width % c.tile_w || c.height % c.tile_h)
                 return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_render_fwd: camera size must be a multiple of its tile");
             uint32_t n = c.width * c.height;
-            render_fwd_kernel<<<(n + 255) / 256, 256, 0, s>>>(w, c, n, max_iters, light_dir[0], light_dir[1],
+            render_fwd_kernel<<<(n + 63) / 64, 64, 0, s>>>(w, c, n, max_iters, light_dir[0], light_dir[1],
                                                               light_dir[2], intensity, albedo, image + base * 3,
                                                               saved_voxel + base, saved_weight + base);
             VRT_HIP(hipGetLastError());

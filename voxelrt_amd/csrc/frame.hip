@@ -291,11 +291,12 @@ extern "C" int vrt_cast_frame(const vrt_world* world, const vrt_camera* cam, uin
         WorldView w = world->view();
         uint32_t sched = o.schedule == VRT_SCHED_AUTO ? VRT_SCHED_GRID : o.schedule;
         if (sched == VRT_SCHED_GRID) {
-            uint32_t blocks = (a.n + 255) / 256;
+            // one wave per workgroup, as the plain cast's grid
+            uint32_t blocks = (a.n + 63) / 64;
             if (o.counters)
-                frame_grid_kernel<true><<<blocks, 256, 0, s>>>(w, a, o.counters);
+                frame_grid_kernel<true><<<blocks, 64, 0, s>>>(w, a, o.counters);
             else
-                frame_grid_kernel<false><<<blocks, 256, 0, s>>>(w, a, nullptr);
+                frame_grid_kernel<false><<<blocks, 64, 0, s>>>(w, a, nullptr);
             VRT_HIP(hipGetLastError());
             return VRT_OK;
         }

@@ -52,17 +52,17 @@ extern "C" int vrt_cast_shader(const vrt_world* world, const vrt_rays* rays, con
         uint32_t cap = opts->max_iters ? opts->max_iters : (coarse ? 96u : 256u);
         ShaderView sv = world->shader_view();
         hipStream_t s = (hipStream_t)stream;
-        uint32_t blocks = (r.n + 255) / 256;
+        uint32_t blocks = (r.n + 63) / 64;
         if (coarse) {
             if (opts->ray_cell_lut)
-                shader_cast_kernel<true, true><<<blocks, 256, 0, s>>>(sv, r, h, cap);
+                shader_cast_kernel<true, true><<<blocks, 64, 0, s>>>(sv, r, h, cap);
             else
-                shader_cast_kernel<true, false><<<blocks, 256, 0, s>>>(sv, r, h, cap);
+                shader_cast_kernel<true, false><<<blocks, 64, 0, s>>>(sv, r, h, cap);
         } else {
             if (opts->ray_cell_lut)
-                shader_cast_kernel<false, true><<<blocks, 256, 0, s>>>(sv, r, h, cap);
+                shader_cast_kernel<false, true><<<blocks, 64, 0, s>>>(sv, r, h, cap);
             else
-                shader_cast_kernel<false, false><<<blocks, 256, 0, s>>>(sv, r, h, cap);
+                shader_cast_kernel<false, false><<<blocks, 64, 0, s>>>(sv, r, h, cap);
         }
         VRT_HIP(hipGetLastError());
         return VRT_OK;
