@@ -15,6 +15,7 @@ on the launch stream) and the reference CPU path (the oracle's SIMD port of
 CpuRenderer.cpp:172-224) timed on this host's cores on the same ray buffers.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--tile 8x8] [--schedule grid|persistent]
+                    [--primary-order|--bounce-order|--fused-order auto|forward|reverse]
 """
 from __future__ import annotations
 
@@ -211,6 +212,9 @@ def main() -> int:
     ap.add_argument("--schedule", default="grid", choices=["grid", "persistent"])
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--refill", type=int, default=0)
+    for launch in ("primary", "bounce", "fused"):
+        ap.add_argument(f"--{launch}-order", default="auto", choices=["auto", "forward", "reverse"],
+                        help=f"grid block order of the {launch} launch; auto: timed after warm-up")
     ap.add_argument("--cpu-runs", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
@@ -263,21 +267,25 @@ def main() -> int:
     bufs_b = bounce_rays.buffers()
     sh = stream.cuda_stream
 
-    def opts(counters=None, cnt=None):
+    order_arg = {"primary": args.primary_order, "bounce": args.bounce_order, "fused": args.fused_order}
+    order = {k: L.VRT_ORDER_REVERSE if v == "reverse" else L.VRT_ORDER_FORWARD for k, v in order_arg.items()}
+
+    def opts(counters=None, cnt=None, which="primary"):
         return L.CastOpts(sched, args.refill, args.chunk, 0, counters.data_ptr() if counters is not None else 0,
-                          cnt.data_ptr() if cnt is not None else 0)
+                          cnt.data_ptr() if cnt is not None else 0, 0, 0, 0, 0, order[which])
 
-    o_plain = opts()
-    o_count = opts(cnt=count)
+    cur = {"primary": opts(), "bounce": opts(cnt=count, which="bounce")}  # the timed launches' options
 
-    def cast_primary(o=o_plain):
+    def cast_primary(o=None):
+        o = cur["primary"] if o is None else o
         L.check(lib.vrt_cast_ex(world.handle, C.byref(r_prim), W.MAX_ITERS, C.byref(h_prim), C.byref(o), sh))
 
     def gen_bounce():
         L.check(lib.vrt_bounce_generate(C.byref(r_prim), C.byref(h_prim), cam.c.width, W.BOUNCE_SEED,
                                         C.byref(bufs_b), src.data_ptr(), count.data_ptr(), scratch.data_ptr(), sh))
 
-    def cast_bounce(o=o_count):
+    def cast_bounce(o=None):
+        o = cur["bounce"] if o is None else o
         L.check(lib.vrt_cast_ex(world.handle, C.byref(r_b), W.MAX_ITERS, C.byref(h_b), C.byref(o), sh))
 
     if args.copy_mb:
@@ -295,13 +303,28 @@ def main() -> int:
         cast_bounce()
     torch.cuda.synchronize()
     nb = int(count.item())
+    if "auto" in (args.primary_order, args.bounce_order) and sched == L.VRT_SCHED_GRID:
+        # the grid's block order decides which waves form each launch's tail; results are the
+        # same: time whole steps with each pair of orders and keep the fastest pair
+        trial = {}
+        names = ("forward", "reverse")
+        both = (L.VRT_ORDER_FORWARD, L.VRT_ORDER_REVERSE)
+        for po in both if args.primary_order == "auto" else (order["primary"],):
+            for bo in both if args.bounce_order == "auto" else (order["bounce"],):
+                order["primary"], order["bounce"] = po, bo
+                cur["primary"], cur["bounce"] = opts(), opts(cnt=count, which="bounce")
+                trial[(po, bo)] = _time_ms(lambda: (cast_primary(), gen_bounce(), cast_bounce()), 20)
+        order["primary"], order["bounce"] = min(trial, key=trial.get)
+        cur["primary"], cur["bounce"] = opts(), opts(cnt=count, which="bounce")
+        log("block order (primary, bounce): " + ", ".join(f"{names[p]}/{names[b]} {t:.4f} ms"
+                                                          for (p, b), t in trial.items()))
 
     # counters (same traversal, counting kernel variant; outside the timed region)
     cp = torch.zeros(4, dtype=torch.int64, device=dev)
     cb = torch.zeros(4, dtype=torch.int64, device=dev)
     cast_primary(opts(counters=cp))
     gen_bounce()
-    cast_bounce(opts(counters=cb, cnt=count))
+    cast_bounce(opts(counters=cb, cnt=count, which="bounce"))
     torch.cuda.synchronize()
     cp_h, cb_h = cp.cpu().numpy(), cb.cpu().numpy()
     assert int(cp_h[0]) == n and int(cb_h[0]) == nb, (cp_h, cb_h, n, nb)
@@ -356,7 +379,8 @@ def main() -> int:
     if traffic_file.exists():
         try:
             tr = json.loads(traffic_file.read_text())
-            if tr.get("tile") == args.tile and tr.get("schedule") == args.schedule:
+            orders = {k: ("reverse" if v else "forward") for k, v in order.items() if k != "fused"}
+            if tr.get("tile") == args.tile and tr.get("schedule") == args.schedule and tr.get("order") == orders:
                 roofline["traffic"] = tr.get("hbm_bytes_per_launch")
         except (ValueError, OSError):
             pass
@@ -376,12 +400,19 @@ def main() -> int:
     torch.cuda.synchronize()
     src_h = src[:nb].long()
     frame_ok = bool(torch.equal(fp.rec, hits_p.rec) and torch.equal(fb.rec[src_h], hits_b.rec[:nb]))
-    frame_ms = _time_ms(lambda: V.cast_frame(world, cam, W.MAX_ITERS, W.BOUNCE_SEED, schedule=1, out=(fp, fb)), K)
+    if args.fused_order == "auto":
+        ft = {o_try: _time_ms(lambda: V.cast_frame(world, cam, W.MAX_ITERS, W.BOUNCE_SEED, schedule=1,
+                                                   out=(fp, fb), order=o_try), 20)
+              for o_try in (L.VRT_ORDER_FORWARD, L.VRT_ORDER_REVERSE)}
+        order["fused"] = min(ft, key=ft.get)
+    frame_ms = _time_ms(lambda: V.cast_frame(world, cam, W.MAX_ITERS, W.BOUNCE_SEED, schedule=1, out=(fp, fb),
+                                             order=order["fused"]), K)
     fused = {
         "value": round((n + nb) / frame_ms / 1e3, 1),
         "unit": "Mrays/s",
         "ms_per_step": round(frame_ms, 4),
         "kernel": "vrt_cast_frame (grid): primary + bounce per lane",
+        "order": "reverse" if order["fused"] else "forward",
         "parity_with_plain": frame_ok,
     }
     log(f"fused frame {fused['value']} Mrays/s ({frame_ms:.3f} ms), parity with the plain pipeline {frame_ok}")
@@ -444,6 +475,7 @@ def main() -> int:
             "max_iters": W.MAX_ITERS,
             "tile": args.tile,
             "schedule": args.schedule,
+            "order": {k: ("reverse" if v else "forward") for k, v in order.items() if k != "fused"},
             "parallelism": "single GPU",
         },
         "roofline": roofline,

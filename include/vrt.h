@@ -186,6 +186,13 @@ typedef enum vrt_cast_schedule {
                                   (long-tailed ray sets) */
 } vrt_cast_schedule;
 
+/* Order in which the grid schedule dispatches its blocks of 64 rays (results are identical). */
+typedef enum vrt_cast_order {
+    VRT_ORDER_FORWARD = 0,     /* block 0 first */
+    VRT_ORDER_REVERSE = 1      /* last block first: cheap rays at the start of the buffer (a camera's
+                                  sky rows) then run in the launch's tail, where the chip drains */
+} vrt_cast_order;
+
 typedef struct vrt_cast_opts {
     uint32_t schedule;          /* vrt_cast_schedule */
     uint32_t refill;            /* persistent: free lanes that trigger a retire/refill, 1..64 (0 = 16) */
@@ -201,6 +208,7 @@ typedef struct vrt_cast_opts {
                                    per-device buffer owned by the library (multipass casts on one device
                                    must then not overlap across streams) */
     size_t scratch_bytes;
+    uint32_t order;             /* grid: vrt_cast_order (0 = forward) */
 } vrt_cast_opts;
 
 int vrt_cast_ex(const vrt_world* world, const vrt_rays* rays, uint32_t max_iters,
@@ -323,6 +331,7 @@ typedef struct vrt_frame_opts {
     uint32_t refill;              /* persistent: parked lanes that trigger a retire/refill (0 = 16) */
     uint32_t chunk;               /* persistent: pixels per queue grab, multiple of 64 (0 = 256) */
     vrt_cast_counters* counters;  /* optional device totals over primary and bounce rays */
+    uint32_t order;               /* grid: vrt_cast_order over blocks of 64 pixels (0 = forward) */
 } vrt_frame_opts;
 
 int vrt_cast_frame(const vrt_world* world, const vrt_camera* cam, uint32_t max_iters, uint32_t seed,

@@ -119,6 +119,34 @@ def test_tiny_iteration_caps(vrt, oracle_mod, cuda, max_iters):
             gpu_cast(world, o, d, (0, 0, 0), 0, 5, 0)
 
 
+def test_reverse_block_order(vrt, oracle_mod, cuda):
+    """The grid's reverse block order writes the same hits (also with a device ray count and a
+    partial last block)."""
+    import voxelrt_amd as V
+    from voxelrt_amd import RayBatch
+
+    vox = blob_world(128, seed=41, density=0.01)
+    world, ow = pair_from_dense(vrt, oracle_mod, vox)
+    o, d = random_rays(10001, 128, seed=42)
+    ref = ow.cast(o, d, (0, 0, 0), 128, with_pos=True)
+    rb = RayBatch.from_numpy(o, d, (0, 0, 0))
+    for order in (V.VRT_ORDER_FORWARD, V.VRT_ORDER_REVERSE):
+        h = world.cast(rb, 128, with_pos=True, order=order)
+        torch.cuda.synchronize()
+        assert_hits_equal(h.t.cpu().numpy(), h.rec.cpu().numpy(), ref["t"], ref["rec"], f"order={order}")
+        hit = (ref["rec"][:, 3] & 0xFF) != 0
+        assert np.array_equal(h.pos.cpu().numpy()[:, hit].view(np.uint32), ref["pos"][:, hit].view(np.uint32))
+        cnt = torch.tensor([7777], dtype=torch.int32, device="cuda")
+        h2 = V.HitBatch.empty(rb.n)
+        h2.rec.fill_(-7)
+        world.cast(rb, 128, out=h2, count=cnt, order=order)
+        torch.cuda.synchronize()
+        rec2 = h2.rec.cpu().numpy()
+        assert np.array_equal(rec2[:7777], ref["rec"][:7777]) and np.all(rec2[7777:] == -7)
+    with pytest.raises(V.VrtError, match="order"):
+        world.cast(rb, 128, order=2)
+
+
 def test_special_rays(vrt, oracle_mod, cuda):
     """Exact zeros (inf/NaN side distances), denormal and huge components, far origins (floor2i
     out of range), origins on voxel faces, empty and solid worlds."""

@@ -9,7 +9,8 @@ import torch
 
 pytestmark = pytest.mark.gpu
 
-SCHEDS = [(1, 0, 0), (2, 16, 256), (2, 8, 64), (2, 32, 1024), (2, 64, 128)]  # (schedule, refill, chunk)
+# (schedule, refill, chunk, order): grid in both block orders, persistent with several refills
+SCHEDS = [(1, 0, 0, 0), (1, 0, 0, 1), (2, 16, 256, 0), (2, 8, 64, 0), (2, 32, 1024, 0), (2, 64, 128, 0)]
 
 
 @pytest.mark.parametrize("config", ["c1", "c2"])
@@ -33,12 +34,12 @@ def test_frame_equals_separate_pipeline(vrt, cuda, config):
     c_ref = V.new_counters()
     world.cast(prim, W.MAX_ITERS, counters=c_ref)
     world.cast(br, W.MAX_ITERS, count=cnt, counters=c_ref)
-    for sched, refill, chunk in SCHEDS:
+    for sched, refill, chunk, order in SCHEDS:
         ctr = V.new_counters()
         hp, hb = V.cast_frame(world, cam, W.MAX_ITERS, 9, with_pos=True, schedule=sched, refill=refill, chunk=chunk,
-                              counters=ctr)
+                              counters=ctr, order=order)
         torch.cuda.synchronize()
-        what = f"{config} sched={sched} refill={refill} chunk={chunk}"
+        what = f"{config} sched={sched} refill={refill} chunk={chunk} order={order}"
         assert torch.equal(hp.rec, hp_ref.rec), what
         assert torch.equal(hp.t, hp_ref.t), what
         hit = (hp.rec[:, 3] & 0xFF) != 0

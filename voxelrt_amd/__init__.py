@@ -4,7 +4,7 @@ The product is libvoxelrt_hip.so (C ABI in include/vrt.h); this package holds
 its sources (csrc/), the build script and a thin Python mirror of the
 reference's voxel-map / ray-cast interface used by tests and benchmarks.
 """
-from ._lib import VrtError, CapacityError, lib, LIB_PATH  # noqa: F401
+from ._lib import VrtError, CapacityError, lib, LIB_PATH, VRT_ORDER_FORWARD, VRT_ORDER_REVERSE  # noqa: F401
 from .api import (  # noqa: F401
     Scene,
     World,

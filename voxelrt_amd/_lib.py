@@ -24,6 +24,9 @@ VRT_SCHED_MULTIPASS = 3
 VRT_SCHED_REPACK = 4
 VRT_SCHED_LANES = 5
 
+VRT_ORDER_FORWARD = 0
+VRT_ORDER_REVERSE = 1
+
 VRT_SEG_CAMERA = 0
 VRT_SEG_RAYS = 1
 
@@ -86,7 +89,7 @@ class CastCounters(C.Structure):
 class CastOpts(C.Structure):
     _fields_ = [("schedule", u32), ("refill", u32), ("chunk", u32), ("waves_per_cu", u32),
                 ("counters", P), ("count", P), ("pass_iters", u32), ("passes", u32), ("scratch", P),
-                ("scratch_bytes", C.c_size_t)]
+                ("scratch_bytes", C.c_size_t), ("order", u32)]
 
 
 class HitF64(C.Structure):
@@ -120,7 +123,7 @@ class GBuffer(C.Structure):
 
 
 class FrameOpts(C.Structure):
-    _fields_ = [("schedule", u32), ("refill", u32), ("chunk", u32), ("counters", P)]
+    _fields_ = [("schedule", u32), ("refill", u32), ("chunk", u32), ("counters", P), ("order", u32)]
 
 
 VRT_HIT_NO_RAY = 1 << 14

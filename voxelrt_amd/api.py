@@ -304,16 +304,17 @@ class World:
              chunk: int = 0, waves_per_cu: int = 0, counters: torch.Tensor | None = None,
              out: HitBatch | None = None, count: torch.Tensor | None = None, refill: int = 0,
              pass_iters: int = 0, passes: int = 0, scratch: torch.Tensor | None = None,
-             stream=None) -> HitBatch:
+             order: int = 0, stream=None) -> HitBatch:
         """Trace rays (RayCast, CpuRenderer.cpp:172-224).  count: optional device int32 holding the
-        number of rays to trace (rays.n is then the capacity)."""
+        number of rays to trace (rays.n is then the capacity); order: the grid's block order
+        (VRT_ORDER_FORWARD / VRT_ORDER_REVERSE, same results)."""
         hits = out if out is not None else HitBatch.empty(rays.n, with_pos, rays.o.device)
         r = rays.to_c()
         h = hits.to_c()
         opts = L.CastOpts(schedule, refill, chunk, waves_per_cu, counters.data_ptr() if counters is not None else 0,
                           count.data_ptr() if count is not None else 0, pass_iters, passes,
                           scratch.data_ptr() if scratch is not None else 0,
-                          scratch.numel() * scratch.element_size() if scratch is not None else 0)
+                          scratch.numel() * scratch.element_size() if scratch is not None else 0, order)
         L.check(L.lib().vrt_cast_ex(self._h, C.byref(r), max_iters, C.byref(h), C.byref(opts),
                                     _stream_handle(stream)))
         return hits
@@ -353,15 +354,16 @@ class World:
 
 def cast_frame(world: "World", cam: "CameraSpec", max_iters: int = 128, seed: int = 1, with_pos: bool = False,
                schedule: int = 1, refill: int = 0, chunk: int = 0, counters: torch.Tensor | None = None,
-               out=None, stream=None):
-    """vrt_cast_frame: (primary HitBatch, bounce HitBatch indexed by pixel)."""
+               out=None, order: int = 0, stream=None):
+    """vrt_cast_frame: (primary HitBatch, bounce HitBatch indexed by pixel); order: the grid's
+    block order (VRT_ORDER_FORWARD / VRT_ORDER_REVERSE, same results)."""
     n = cam.n
     if out is None:
         dev = "cuda"
         out = (HitBatch.empty(n, with_pos, dev), HitBatch.empty(n, False, dev))
     hp, hb = out
     a, b = hp.to_c(), hb.to_c()
-    o = L.FrameOpts(schedule, refill, chunk, counters.data_ptr() if counters is not None else 0)
+    o = L.FrameOpts(schedule, refill, chunk, counters.data_ptr() if counters is not None else 0, order)
     L.check(L.lib().vrt_cast_frame(world.handle, C.byref(cam.c), max_iters, seed, C.byref(a), C.byref(b),
                                    C.byref(o), _stream_handle(stream)))
     return hp, hb

@@ -73,10 +73,17 @@ __device__ __forceinline__ void trace_full(const WorldView& w, RayState& st, uin
 // slower: the eighths differ in work and the XCDs finish unevenly.
 constexpr uint32_t kGridThreads = 64;
 
+// Ray blocks are dispatched in order, or last block first (reverse): the results
+// are the same, but the waves dispatched last make up the launch's tail, where the
+// chip drains.  A camera's last rows (the ground, with descents and hits) cost the
+// most per wave and its first rows (sky) the least, so the reverse order leaves cheap
+// waves for the tail: 0.118 against 0.139 ms for the 1920x1080 primary rays.  A random
+// block order (0.145 ms) loses the L2 reuse between neighbouring tiles.
 template <bool COUNT>
 __global__ __launch_bounds__(kGridThreads) void cast_grid_kernel(WorldView w, RaysIn r, HitsOut h, uint32_t max_iters,
-                                                                 vrt_cast_counters* ctr) {
-    uint32_t i = blockIdx.x * kGridThreads + threadIdx.x;
+                                                                 vrt_cast_counters* ctr, uint32_t reverse) {
+    uint32_t blk = reverse ? gridDim.x - 1u - blockIdx.x : blockIdx.x;
+    uint32_t i = blk * kGridThreads + threadIdx.x;
     bool valid = i < ray_count(r);
     RayState st;
     float t = 0.0f;
@@ -713,11 +720,13 @@ int launch_cast(const vrt_world* world, const RaysIn& r, const HitsOut& h, uint3
     if (r.n == 0) return VRT_OK;
     uint32_t sched = o.schedule == VRT_SCHED_AUTO ? VRT_SCHED_GRID : o.schedule;
     if (sched == VRT_SCHED_GRID) {
+        if (o.order > VRT_ORDER_REVERSE) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_cast_ex: unknown order");
         uint32_t blocks = (uint32_t)(((uint64_t)r.n + kGridThreads - 1) / kGridThreads);
+        uint32_t rev = o.order == VRT_ORDER_REVERSE ? 1u : 0u;
         if (o.counters)
-            cast_grid_kernel<true><<<blocks, kGridThreads, 0, s>>>(w, r, h, max_iters, o.counters);
+            cast_grid_kernel<true><<<blocks, kGridThreads, 0, s>>>(w, r, h, max_iters, o.counters, rev);
         else
-            cast_grid_kernel<false><<<blocks, kGridThreads, 0, s>>>(w, r, h, max_iters, nullptr);
+            cast_grid_kernel<false><<<blocks, kGridThreads, 0, s>>>(w, r, h, max_iters, nullptr, rev);
         VRT_HIP(hipGetLastError());
         return VRT_OK;
     }

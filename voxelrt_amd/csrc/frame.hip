@@ -111,8 +111,11 @@ __device__ __forceinline__ void finish_trace(const WorldView& w, const RayState&
 }
 
 template <bool COUNT>
-__global__ __launch_bounds__(256) void frame_grid_kernel(WorldView w, FrameArgs a, vrt_cast_counters* ctr) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void frame_grid_kernel(WorldView w, FrameArgs a, vrt_cast_counters* ctr,
+                                                         uint32_t reverse) {
+    // blocks of pixels in order or last block first, as the plain cast's grid
+    uint32_t blk = reverse ? gridDim.x - 1u - blockIdx.x : blockIdx.x;
+    uint32_t i = blk * blockDim.x + threadIdx.x;
     bool valid = i < a.n;
     unsigned long long c[4] = {0, 0, 0, 0};
     RayState st;
@@ -292,11 +295,13 @@ extern "C" int vrt_cast_frame(const vrt_world* world, const vrt_camera* cam, uin
         uint32_t sched = o.schedule == VRT_SCHED_AUTO ? VRT_SCHED_GRID : o.schedule;
         if (sched == VRT_SCHED_GRID) {
             // one wave per workgroup, as the plain cast's grid
+            if (o.order > VRT_ORDER_REVERSE) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_cast_frame: unknown order");
             uint32_t blocks = (a.n + 63) / 64;
+            uint32_t rev = o.order == VRT_ORDER_REVERSE ? 1u : 0u;
             if (o.counters)
-                frame_grid_kernel<true><<<blocks, 64, 0, s>>>(w, a, o.counters);
+                frame_grid_kernel<true><<<blocks, 64, 0, s>>>(w, a, o.counters, rev);
             else
-                frame_grid_kernel<false><<<blocks, 64, 0, s>>>(w, a, nullptr);
+                frame_grid_kernel<false><<<blocks, 64, 0, s>>>(w, a, nullptr, rev);
             VRT_HIP(hipGetLastError());
             return VRT_OK;
         }
