@@ -363,7 +363,7 @@ typedef struct vrt_render_opts {
     uint32_t max_bounces;   /* u_MaxBounces, 0..16 */
     uint32_t frame;         /* u_FrameNo: seeds the bounce directions */
     uint32_t ray_cell_lut;  /* u_UseAnisotropicLods: 0 or 1 */
-    uint32_t reserved;      /* 0 */
+    uint32_t order;         /* vrt_cast_order of the 8x8 pixel tiles (same images) */
     float sky[3];           /* sky radiance */
     float near_z, far_z;    /* depth projection; both 0 = glim::Camera's 0.01 / 1000 */
 } vrt_render_opts;

@@ -145,6 +145,7 @@ public:
     float Sky[3] = {0.6f, 0.75f, 1.0f};
     float NearZ = 0.01f, FarZ = 1000.0f;
     uint32_t FrameNo = 0;
+    bool LastTileFirst = false;        // tile dispatch order (same images; see vrt_cast_order)
 
     void RenderFrame(const vrt_camera& cam, vrt_gbuffer& out, void* stream = nullptr) {
         vrt_render_opts o{};
@@ -154,6 +155,7 @@ public:
         for (int c = 0; c < 3; c++) o.sky[c] = Sky[c];
         o.near_z = NearZ;
         o.far_z = FarZ;
+        o.order = LastTileFirst ? VRT_ORDER_REVERSE : VRT_ORDER_FORWARD;
         check(vrt_render_frame(world_, &cam, &o, &out, stream));
     }
 

@@ -24,8 +24,8 @@ def random_palette(seed: int) -> np.ndarray:
     return pal
 
 
-def render_both(vrt, world, ow, cam, bounces, frame, lut):
-    g = vrt.render_frame(world, cam, bounces, frame, lut, SKY)
+def render_both(vrt, world, ow, cam, bounces, frame, lut, order=0):
+    g = vrt.render_frame(world, cam, bounces, frame, lut, SKY, order=order)
     torch.cuda.synchronize()
     o, d = cam.rays_host()
     sun, pa, pb = render_constants()
@@ -73,14 +73,18 @@ def test_gbuffer_odd_size_and_errors(vrt, oracle_mod, cuda):
     world.set_palette(pal)
     ow.set_palette(pal)
     cam = W.c1_camera(tile=(1, 1), width=37, height=21)
-    g, ref = render_both(vrt, world, ow, cam, 1, 2, True)
-    assert np.array_equal(g.albedo_normal.cpu().numpy().view(np.uint32), ref["albedo_normal"])
-    assert np.array_equal(g.irradiance.cpu().numpy().view(np.uint32), ref["irradiance"].view(np.uint32))
-    assert np.array_equal(g.depth.cpu().numpy().view(np.uint32), ref["depth"].view(np.uint32))
+    for order in (vrt.VRT_ORDER_FORWARD, vrt.VRT_ORDER_REVERSE):  # partial tiles in both corners
+        g, ref = render_both(vrt, world, ow, cam, 1, 2, True, order)
+        assert np.array_equal(g.albedo_normal.cpu().numpy().view(np.uint32), ref["albedo_normal"])
+        assert np.array_equal(g.irradiance.cpu().numpy().view(np.uint32), ref["irradiance"].view(np.uint32))
+        assert np.array_equal(g.depth.cpu().numpy().view(np.uint32), ref["depth"].view(np.uint32))
     o = L.RenderOpts(17, 0, 0, 0, (C.c_float * 3)(*SKY), 0.0, 0.0)
     c = g.to_c()
     assert vrt.lib().vrt_render_frame(world.handle, C.byref(cam.c), C.byref(o), C.byref(c), None) == \
         L.VRT_ERR_INVALID_ARGUMENT
     o = L.RenderOpts(1, 0, 0, 0, (C.c_float * 3)(*SKY), 5.0, 1.0)
+    assert vrt.lib().vrt_render_frame(world.handle, C.byref(cam.c), C.byref(o), C.byref(c), None) == \
+        L.VRT_ERR_INVALID_ARGUMENT
+    o = L.RenderOpts(1, 0, 0, 2, (C.c_float * 3)(*SKY), 0.0, 0.0)  # unknown tile order
     assert vrt.lib().vrt_render_frame(world.handle, C.byref(cam.c), C.byref(o), C.byref(c), None) == \
         L.VRT_ERR_INVALID_ARGUMENT

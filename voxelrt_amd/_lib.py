@@ -114,7 +114,7 @@ class CvoxInfo(C.Structure):
 
 
 class RenderOpts(C.Structure):
-    _fields_ = [("max_bounces", u32), ("frame", u32), ("ray_cell_lut", u32), ("reserved", u32), ("sky", f32 * 3),
+    _fields_ = [("max_bounces", u32), ("frame", u32), ("ray_cell_lut", u32), ("order", u32), ("sky", f32 * 3),
                 ("near_z", f32), ("far_z", f32)]
 
 

@@ -384,10 +384,11 @@ class GBufferImages:
 
 def render_frame(world: "World", cam: "CameraSpec", max_bounces: int = 1, frame: int = 0,
                  ray_cell_lut: bool = False, sky=(0.6, 0.75, 1.0), near_z: float = 0.0, far_z: float = 0.0,
-                 out: GBufferImages | None = None, stream=None) -> GBufferImages:
-    """vrt_render_frame: one frame of the GPU renderer (VoxelRender.comp:28-94) into G-buffer images."""
+                 out: GBufferImages | None = None, order: int = 0, stream=None) -> GBufferImages:
+    """vrt_render_frame: one frame of the GPU renderer (VoxelRender.comp:28-94) into G-buffer images;
+    order: the 8x8 tiles in row order or last tile first (VRT_ORDER_*, same images)."""
     g = out if out is not None else GBufferImages(cam.c.width, cam.c.height)
-    o = L.RenderOpts(max_bounces, frame, int(ray_cell_lut), 0, (C.c_float * 3)(*sky), near_z, far_z)
+    o = L.RenderOpts(max_bounces, frame, int(ray_cell_lut), order, (C.c_float * 3)(*sky), near_z, far_z)
     c = g.to_c()
     L.check(L.lib().vrt_render_frame(world.handle, C.byref(cam.c), C.byref(o), C.byref(c), _stream_handle(stream)))
     return g

@@ -44,8 +44,12 @@ __device__ __forceinline__ void normal_of(uint32_t code, float N[3]) {
 __device__ __forceinline__ uint32_t unorm8(float x) { return (uint32_t)(fminf(fmaxf(x, 0.0f), 1.0f) * 255.0f + 0.5f); }
 
 template <bool LUT>
-__global__ __launch_bounds__(64) void render_kernel(ShaderView sv, RenderArgs a) {
-    uint32_t px = blockIdx.x * 8 + (threadIdx.x & 7u), py = blockIdx.y * 8 + (threadIdx.x >> 3);
+__global__ __launch_bounds__(64) void render_kernel(ShaderView sv, RenderArgs a, uint32_t reverse) {
+    // 8x8 tiles in row order, or last tile first (the image's cheap top rows then run in
+    // the launch's tail; the pixels are the same)
+    uint32_t tx = reverse ? gridDim.x - 1u - blockIdx.x : blockIdx.x;
+    uint32_t ty = reverse ? gridDim.y - 1u - blockIdx.y : blockIdx.y;
+    uint32_t px = tx * 8 + (threadIdx.x & 7u), py = ty * 8 + (threadIdx.x >> 3);
     if (px >= a.cam.width || py >= a.cam.height) return;
     const uint32_t i = py * a.cam.width + px;
     const int32_t* W = a.cam.world_origin;
@@ -158,8 +162,8 @@ extern "C" int vrt_render_frame(const vrt_world* world, const vrt_camera* cam, c
         if ((uint64_t)cam->width * cam->height > 0x7FFFFFFFull)
             return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_render_frame: too many pixels");
         if (opts->max_bounces > 16) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_render_frame: max_bounces must be 0..16");
-        if (opts->ray_cell_lut > 1 || opts->reserved != 0)
-            return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_render_frame: ray_cell_lut must be 0 or 1, reserved 0");
+        if (opts->ray_cell_lut > 1 || opts->order > VRT_ORDER_REVERSE)
+            return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_render_frame: ray_cell_lut and order must be 0 or 1");
         float near_z = opts->near_z, far_z = opts->far_z;
         if (near_z == 0.0f && far_z == 0.0f) {
             near_z = 0.01f;
@@ -187,10 +191,11 @@ extern "C" int vrt_render_frame(const vrt_world* world, const vrt_camera* cam, c
         ShaderView sv = world->shader_view();
         dim3 grid((cam->width + 7) / 8, (cam->height + 7) / 8);
         hipStream_t s = (hipStream_t)stream;
+        uint32_t rev = opts->order == VRT_ORDER_REVERSE ? 1u : 0u;
         if (opts->ray_cell_lut)
-            render_kernel<true><<<grid, 64, 0, s>>>(sv, a);
+            render_kernel<true><<<grid, 64, 0, s>>>(sv, a, rev);
         else
-            render_kernel<false><<<grid, 64, 0, s>>>(sv, a);
+            render_kernel<false><<<grid, 64, 0, s>>>(sv, a, rev);
         VRT_HIP(hipGetLastError());
         return VRT_OK;
     });
