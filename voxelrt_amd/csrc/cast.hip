@@ -3,12 +3,18 @@
 //
 // vrt_cast replaces the reference's RayCast (src/VoxelRT/CpuRenderer.cpp:172-224)
 // for a buffer of rays; vrt_cast_one_f64 replaces VoxelMap::RayCast
-// (src/VoxelRT/VoxelMap.cpp:140-170).  Two schedules compute identical results:
-//   GRID        one lane per ray, the grid covers n rays;
+// (src/VoxelRT/VoxelMap.cpp:140-170).  Every schedule computes identical results:
+//   GRID        one lane per ray, one wave per workgroup, the grid covers n rays
+//               (blocks of 64 rays in order or last block first); the default;
 //   PERSISTENT  a grid sized to the device; each wave takes chunks of rays from
 //               a global counter into a wave-local pool and refills lanes whose
 //               ray finished (ballot + mbcnt rank), so long rays do not keep
-//               the other 63 lanes of their wave idle.
+//               the other 63 lanes of their wave idle;
+//   LANES       each lane traces K rays back to back, the waiting ones in LDS;
+//   REPACK      workgroups repack their running rays through LDS every few lookups;
+//   MULTIPASS   passes of a few lookups, running rays compacted between passes.
+// The alternatives to GRID are kept as options: on the measured workloads the
+// hardware dispatcher balances plain waves better than they do.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
