@@ -131,3 +131,29 @@ def test_autograd_function(vrt, cuda, c5_small):
     err = (a.grad - manual).abs().max().item()
     assert err <= 1e-4 * manual.abs().max().item()
     assert a.grad.shape == albedo.shape
+
+
+def test_forward_many_views_one_launch(vrt, cuda, c5_small):
+    """18 views of uneven sizes (pixel counts that are not multiples of 64, more views than one
+    launch takes): the same pixels, ids and weights as rendering each view on its own."""
+    import math
+
+    import voxelrt_amd as V
+    from voxelrt_amd import workloads as W
+
+    world, scene, cams, albedo, nvox = c5_small
+    views = []
+    for k in range(18):
+        a = 2 * math.pi * k / 18
+        pos = (256 + 180 * math.sin(a) + 0.13, 400.33, 256 + 180 * math.cos(a) + 0.41)
+        w, h = 40 + 7 * k, 24 + 3 * (k % 5)
+        views.append(V.CameraSpec(pos, math.degrees(a) + 180.0, -30.0, w, h, tile=(1, 1)))
+    image, sv, sw = V.render_saved(world, albedo, views, W.MAX_ITERS)
+    base = 0
+    for cam in views:
+        i1, v1, w1 = V.render_saved(world, albedo, [cam], W.MAX_ITERS)
+        n = cam.n
+        assert torch.equal(image[base:base + n], i1) and torch.equal(sv[base:base + n], v1)
+        assert torch.equal(sw[base:base + n], w1)
+        base += n
+    assert base == image.shape[0] and bool((sv >= 0).any())

@@ -12,24 +12,44 @@
 
 #include <cstring>
 
+#include <algorithm>
+
 #include "traverse.h"
 #include "vrt_cast_impl.h"
 
 namespace vrt {
 
-__global__ __launch_bounds__(256) void render_fwd_kernel(WorldView w, vrt_camera cam, uint32_t n, uint32_t max_iters,
-                                                         float lx, float ly, float lz, float intensity,
-                                                         const float* __restrict__ albedo, float* __restrict__ image,
-                                                         uint32_t* __restrict__ saved_voxel,
-                                                         float* __restrict__ saved_weight) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    bool valid = i < n;
+// Up to kFwdViews views per launch, their cameras passed in the kernel arguments: one
+// launch over many small views fills the chip where one launch per view (a 512x512
+// view is half a round of resident waves) leaves it draining after every view.
+constexpr uint32_t kFwdViews = 16;
+struct FwdViews {
+    vrt_camera cam[kFwdViews];
+    uint32_t block_begin[kFwdViews + 1];  // first block of 64 pixels of each view; [nv] = all blocks
+    uint32_t pixel_base[kFwdViews];       // first output pixel of each view
+    uint32_t nv;
+};
+
+__global__ __launch_bounds__(64) void render_fwd_kernel(WorldView w, FwdViews views, uint32_t max_iters,
+                                                        float lx, float ly, float lz, float intensity,
+                                                        const float* __restrict__ albedo, float* __restrict__ image,
+                                                        uint32_t* __restrict__ saved_voxel,
+                                                        float* __restrict__ saved_weight) {
+    // the wave's view (blocks never straddle views)
+    const uint32_t b = blockIdx.x;
+    uint32_t k = 0;
+    while (k + 1 < views.nv && b >= views.block_begin[k + 1]) k++;
+    const vrt_camera& cam = views.cam[k];
+    const uint32_t n = cam.width * cam.height;
+    const uint32_t li = (b - views.block_begin[k]) * 64u + threadIdx.x;  // pixel within the view
+    const bool valid = li < n;
+    const uint32_t i = views.pixel_base[k] + li;                          // output pixel
     RayState st;
     int32_t v[3] = {0, 0, 0};
     bool tame = true;
     if (valid) {
         uint32_t px, py;
-        tile_pixel(i, cam.width, cam.tile_w, cam.tile_h, px, py);
+        tile_pixel(li, cam.width, cam.tile_w, cam.tile_h, px, py);
         float d[3];
         camera_ray(cam.forward, cam.right, cam.up, 1.0f / (float)cam.width, 1.0f / (float)cam.height, px, py, d);
         ray_setup(st, cam.eye[0], cam.eye[1], cam.eye[2], d[0], d[1], d[2], cam.world_origin[0], cam.world_origin[1],
@@ -137,17 +157,32 @@ int vrt_render_fwd(const vrt_world* world, const vrt_camera* cameras, uint32_t n
             return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_render_fwd: world too large for 32-bit voxel ids");
         hipStream_t s = (hipStream_t)stream;
         WorldView w = world->view();
-        uint64_t base = 0;
+        uint64_t total = 0;
         for (uint32_t k = 0; k < nviews; k++) {
             const vrt_camera& c = cameras[k];
             if (!c.width || !c.height || !c.tile_w || !c.tile_h || c.width % c.tile_w || c.height % c.tile_h)
                 return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_render_fwd: camera size must be a multiple of its tile");
-            uint32_t n = c.width * c.height;
-            render_fwd_kernel<<<(n + 63) / 64, 64, 0, s>>>(w, c, n, max_iters, light_dir[0], light_dir[1],
-                                                              light_dir[2], intensity, albedo, image + base * 3,
-                                                              saved_voxel + base, saved_weight + base);
+            total += (uint64_t)c.width * c.height;
+        }
+        if (total > 0xFFFFFFFFull) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_render_fwd: more than 2^32 - 1 pixels");
+        uint32_t base = 0;
+        for (uint32_t k0 = 0; k0 < nviews; k0 += kFwdViews) {
+            FwdViews v{};
+            v.nv = std::min(kFwdViews, nviews - k0);
+            uint64_t blocks = 0;
+            for (uint32_t k = 0; k < v.nv; k++) {
+                const vrt_camera& c = cameras[k0 + k];
+                v.cam[k] = c;
+                v.block_begin[k] = (uint32_t)blocks;
+                v.pixel_base[k] = base;
+                uint32_t n = c.width * c.height;
+                blocks += (n + 63) / 64;
+                base += n;
+            }
+            v.block_begin[v.nv] = (uint32_t)blocks;
+            render_fwd_kernel<<<(uint32_t)blocks, 64, 0, s>>>(w, v, max_iters, light_dir[0], light_dir[1], light_dir[2],
+                                                               intensity, albedo, image, saved_voxel, saved_weight);
             VRT_HIP(hipGetLastError());
-            base += n;
         }
         return VRT_OK;
     });
