@@ -3,8 +3,9 @@
 
 Workload (one "step"): the 1024^3 terrain + caves world (seeded fBm heightmap,
 3D value-noise caves, ~12% of bricks allocated), 1920x1080 primary rays from a
-fixed camera above the terrain, then one cosine-distributed diffuse bounce ray
-per primary hit (seeded PCG), i.e.
+fixed camera above the terrain (their integer world origin at the camera's
+sector corner; --anchor floor puts it at floor(eye)), then one cosine-distributed
+diffuse bounce ray per primary hit (seeded PCG), i.e.
     vrt_cast(primary) -> vrt_bounce_generate -> vrt_cast(bounce, device count)
 Rays, world and output buffers are resident in HBM before timing starts.
 value = rays traced (primary + bounce) per second, in millions.
@@ -16,6 +17,7 @@ CpuRenderer.cpp:172-224) timed on this host's cores on the same ray buffers.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--tile 8x8] [--schedule grid|persistent]
                     [--primary-order|--bounce-order|--fused-order auto|forward|reverse]
+                    [--anchor sector|floor]
 """
 from __future__ import annotations
 
@@ -215,6 +217,9 @@ def main() -> int:
     for launch in ("primary", "bounce", "fused"):
         ap.add_argument(f"--{launch}-order", default="auto", choices=["auto", "forward", "reverse"],
                         help=f"grid block order of the {launch} launch; auto: timed after warm-up")
+    ap.add_argument("--anchor", default="sector", choices=["floor", "sector"],
+                    help="integer world origin of the camera rays: the camera's sector corner (default; the "
+                         "aligned-origin step) or floor(eye)")
     ap.add_argument("--cpu-runs", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
@@ -246,7 +251,7 @@ def main() -> int:
     log(f"  generated in {time.time() - t0:.1f} s, brick fraction {W.brick_fraction(scene):.4f}")
     world = V.World.from_scene(scene)
     info = world.info()
-    cam = W.c2_camera(tile=tile)
+    cam = W.c2_camera(tile=tile, anchor=args.anchor)
     prim = cam.rays_device()
     n = prim.n
     hits_p = V.HitBatch.empty(n, with_pos=True)
@@ -380,7 +385,8 @@ def main() -> int:
         try:
             tr = json.loads(traffic_file.read_text())
             orders = {k: ("reverse" if v else "forward") for k, v in order.items() if k != "fused"}
-            if tr.get("tile") == args.tile and tr.get("schedule") == args.schedule and tr.get("order") == orders:
+            if (tr.get("tile") == args.tile and tr.get("schedule") == args.schedule and tr.get("order") == orders
+                    and tr.get("anchor") == args.anchor):
                 roofline["traffic"] = tr.get("hbm_bytes_per_launch")
         except (ValueError, OSError):
             pass
@@ -476,6 +482,7 @@ def main() -> int:
             "tile": args.tile,
             "schedule": args.schedule,
             "order": {k: ("reverse" if v else "forward") for k, v in order.items() if k != "fused"},
+            "camera_anchor": f"{args.anchor}: world origin {list(prim.world_origin)}, eye {[round(x, 4) for x in cam.c.eye]}",
             "parallelism": "single GPU",
         },
         "roofline": roofline,

@@ -82,6 +82,8 @@ typedef struct vrt_world_info {
     uint32_t num_bricks;           /* bricks currently allocated */
     uint32_t slot_capacity;        /* device slots currently backed by memory */
     uint64_t device_bytes;         /* HBM held by the world buffers */
+    uint32_t slot_high_water;      /* one past the highest allocated slot: per-voxel arrays in
+                                      slot order (the render's albedo) need this many * 512 entries */
 } vrt_world_info;
 
 int vrt_world_create(const vrt_world_desc* desc, void* stream, vrt_world** out);
@@ -443,7 +445,9 @@ int vrt_batch_autotune(const vrt_world* world, const vrt_batch* batch, const cha
 /*
  * Lambertian render of camera views, one primary ray per pixel, no shadows:
  *   radiance_c = albedo[v][c] * max(0, dot(N, L)) * intensity   (0 on a miss)
- * albedo is fp32 RGB per voxel in slot order: index (slot * 512 + voxel) * 3 + c.
+ * albedo is fp32 RGB per voxel in slot order: index (slot * 512 + voxel) * 3 + c; it
+ * holds at least vrt_world_info.slot_high_water * 512 voxels (the library cannot check
+ * this: the array's length is not passed).
  * image: [sum of width*height over views] * 3, views concatenated, pixels in
  * each camera's ray order.  saved_voxel (u32, ~0 on a miss) and saved_weight
  * (f32) are per pixel and are consumed by the backward pass.
@@ -453,7 +457,8 @@ int vrt_render_fwd(const vrt_world* world, const vrt_camera* cameras, uint32_t n
                    const float* albedo, float* image, uint32_t* saved_voxel,
                    float* saved_weight, void* stream);
 
-/* grad_albedo[v][c] += sum over pixels hitting v of grad_image[p][c] * saved_weight[p]. */
+/* grad_albedo[v][c] += sum over pixels hitting v of grad_image[p][c] * saved_weight[p].
+ * grad_albedo has the albedo's length; the world is unchanged since the forward pass. */
 int vrt_render_bwd(const vrt_world* world, uint32_t npixels, const float* grad_image,
                    const uint32_t* saved_voxel, const float* saved_weight,
                    float* grad_albedo, void* stream);

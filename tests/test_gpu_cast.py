@@ -76,7 +76,8 @@ def test_random_world_parity(vrt, oracle_mod, cuda, seed, size, inside, max_iter
     vox = blob_world(size, seed=seed, density=0.01)
     world, ow = pair_from_dense(vrt, oracle_mod, vox)
     o, d = random_rays(30000, size, seed=seed + 50, inside=inside)
-    W = (0, 0, 0) if seed % 2 else (-7, 3, 11)
+    # world origins of each specialization of the step: zero, any, a sector corner
+    W = {1: (0, 0, 0), 2: (-7, 3, 11), 3: (32, -64, 96), 4: (-32, 0, 64)}[seed]
     o = (o - np.array(W, np.float32)[:, None]).astype(np.float32)
     ref = ow.cast(o, d, W, max_iters)
     for sched, chunk in SCHEDULES:

@@ -62,7 +62,8 @@ class WorldDesc(C.Structure):
 
 class WorldInfo(C.Structure):
     _fields_ = [("size_xz", u32), ("size_y", u32), ("sectors_xz", u32), ("sectors_y", u32),
-                ("num_sectors", u32), ("num_bricks", u32), ("slot_capacity", u32), ("device_bytes", u64)]
+                ("num_sectors", u32), ("num_bricks", u32), ("slot_capacity", u32), ("device_bytes", u64),
+                ("slot_high_water", u32)]
 
 
 class WorldBuffers(C.Structure):

@@ -12,6 +12,7 @@ done by libvoxelrt_hip.so through its C ABI.  Every failure raises VrtError.
 from __future__ import annotations
 
 import ctypes as C
+import math
 from dataclasses import dataclass
 
 import numpy as np
@@ -217,13 +218,30 @@ class HitResult:
 
 
 class CameraSpec:
-    """vrt_camera built by vrt_camera_look."""
+    """vrt_camera built by vrt_camera_look.
+
+    anchor: the rays' integer world origin W (eye = pos - W, rounded to f32):
+      "floor"   floor(pos), the reference's choice (eye in [0, 1));
+      "sector"  floor(pos) rounded down to a multiple of 32 per axis, the corner of the camera's
+                sector (eye in [0, 32)): the casts take their aligned-origin step, 3 VALU
+                instructions shorter;
+      "zero"    0, world-space rays: the step also drops the additions of W (6 shorter).
+    The rays are the same world rays up to the f32 rounding of the eye."""
+
+    ANCHORS = ("floor", "sector", "zero")
 
     def __init__(self, pos, yaw_deg: float, pitch_deg: float, width: int, height: int, fov_deg: float = 90.0,
-                 tile=(1, 1)):
+                 tile=(1, 1), anchor: str = "floor"):
+        if anchor not in self.ANCHORS:
+            raise ValueError(f"anchor must be one of {self.ANCHORS}")
         cam = L.Camera()
         L.check(L.lib().vrt_camera_look(C.byref((C.c_double * 3)(*pos)), yaw_deg, pitch_deg, fov_deg, width,
                                         height, tile[0], tile[1], C.byref(cam)))
+        if anchor != "floor":
+            for a in range(3):
+                w = math.floor(pos[a]) & ~31 if anchor == "sector" else 0
+                cam.world_origin[a] = w
+                cam.eye[a] = float(pos[a]) - w  # exact in double; ctypes rounds to f32 like vrt_camera_look
         self.c = cam
 
     @property
@@ -538,6 +556,8 @@ def torch_ext():
 
 def _pack_cameras(cameras) -> torch.Tensor:
     raw = b"".join(bytes(c.c) for c in cameras)
+    if not raw:
+        return torch.empty(0, dtype=torch.uint8)
     return torch.frombuffer(bytearray(raw), dtype=torch.uint8)
 
 
@@ -561,6 +581,11 @@ def render_saved(world: World, albedo: torch.Tensor, cameras, max_iters: int = 1
 def render_backward(world: World, grad_image: torch.Tensor, saved_voxel: torch.Tensor, saved_weight: torch.Tensor,
                     nvox: int, stream=None) -> torch.Tensor:
     """vrt_render_bwd into a fresh zero gradient [nvox, 3]."""
+    need = world.info().slot_high_water * 512
+    if nvox < need:
+        raise ValueError(f"render_backward: nvox {nvox} < slot_high_water * 512 = {need}")
+    if saved_voxel.numel() and grad_image.numel() != 3 * saved_voxel.numel():
+        raise ValueError("render_backward: grad_image must hold 3 values per saved pixel")
     grad = torch.zeros((nvox, 3), dtype=torch.float32, device=grad_image.device)
     g = grad_image.contiguous()
     L.check(L.lib().vrt_render_bwd(world.handle, saved_voxel.numel(), g.data_ptr(), saved_voxel.data_ptr(),

@@ -96,6 +96,15 @@ __device__ __forceinline__ LaneRay lane_ray(const BatchArgs& a, const BatchSegDe
     return lr;
 }
 
+// The origin kind of the segment's rays (the camera's world origin for camera segments).
+__device__ __forceinline__ int segment_origin(const BatchArgs& a, const BatchSegDev& sg) {
+    if (sg.kind == VRT_SEG_CAMERA) {
+        const vrt_camera& c = a.cams[sg.cam];
+        return origin_kind(c.world_origin[0], c.world_origin[1], c.world_origin[2]);
+    }
+    return origin_kind(sg.W[0], sg.W[1], sg.W[2]);
+}
+
 template <int RPL, bool COUNT>
 __device__ __forceinline__ void trace_block(const WorldView& w, const BatchArgs& a, uint32_t block,
                                             vrt_cast_counters* ctr) {
@@ -121,7 +130,7 @@ __device__ __forceinline__ void trace_block(const WorldView& w, const BatchArgs&
     for (int k = 0; k < RPL; k++) hit[k] = false;
     if constexpr (RPL == 1) {
         if (!done[0])
-            hit[0] = ray_trace_wave(w, st[0], a.max_iters, v[0], wave_tame);
+            hit[0] = ray_trace_wave(w, st[0], a.max_iters, v[0], wave_tame, segment_origin(a, sg));
     } else {
         // the lane's rays step in lockstep, so one counter caps them all
         uint32_t unused;
@@ -289,7 +298,6 @@ static int create_plan(const vrt_world* world, const vrt_batch* b, const vrt_bat
     if (b->max_iters == 0 || b->max_iters > 65535) return fail(VRT_ERR_INVALID_ARGUMENT, "batch: max_iters must be 1..65535");
     if (b->nsegments && !b->segments) return fail(VRT_ERR_INVALID_ARGUMENT, "batch: null segment table");
     if (b->ncameras && !b->cameras) return fail(VRT_ERR_INVALID_ARGUMENT, "batch: null camera table");
-    if (!b->hits.t || !b->hits.rec) return fail(VRT_ERR_INVALID_ARGUMENT, "batch: null hit buffers");
     if (((uintptr_t)b->hits.rec & 15) != 0) return fail(VRT_ERR_INVALID_ARGUMENT, "batch: hit records must be 16-byte aligned");
     const uint32_t per_block = 64 * tun->rays_per_lane;
     std::vector<BatchSegDev> segs(b->nsegments);
@@ -317,7 +325,7 @@ static int create_plan(const vrt_world* world, const vrt_batch* b, const vrt_bat
             d.count = c.width * c.height;
             if (g.count_ptr) return fail(VRT_ERR_INVALID_ARGUMENT, "batch: camera segments have a fixed count");
         } else if (g.kind == VRT_SEG_RAYS) {
-            if (!b->rays.ox || !b->rays.oy || !b->rays.oz || !b->rays.dx || !b->rays.dy || !b->rays.dz)
+            if (d.count && (!b->rays.ox || !b->rays.oy || !b->rays.oz || !b->rays.dx || !b->rays.dy || !b->rays.dz))
                 return fail(VRT_ERR_INVALID_ARGUMENT, "batch: ray segments need ray buffers");
         } else {
             return fail(VRT_ERR_INVALID_ARGUMENT, "batch: unknown segment kind");
@@ -327,6 +335,7 @@ static int create_plan(const vrt_world* world, const vrt_batch* b, const vrt_bat
         rays += d.count;
         if (blocks > 0x7FFFFFFFull) return fail(VRT_ERR_INVALID_ARGUMENT, "batch: too many rays");
     }
+    if (rays && (!b->hits.t || !b->hits.rec)) return fail(VRT_ERR_INVALID_ARGUMENT, "batch: null hit buffers");
     auto* p = new vrt_batch_plan();
     p->world = world;
     p->device = world->device;
@@ -439,6 +448,7 @@ int vrt_batch_launch(vrt_batch_plan* plan, vrt_cast_counters* counters, void* st
 int vrt_batch_graph_launch(vrt_batch_plan* plan, void* stream) {
     return guarded([&]() -> int {
         if (!plan) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_batch_graph_launch: null plan");
+        if (plan->args.nblocks == 0) return VRT_OK;  // nothing to cast, nothing to capture
         if (!plan->graph_exec) {
             // capture on a private stream (the null stream cannot be captured)
             VRT_HIP(hipStreamCreateWithFlags(&plan->capture_stream, hipStreamNonBlocking));

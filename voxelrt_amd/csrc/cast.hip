@@ -61,9 +61,9 @@ __device__ __forceinline__ void store_result(const HitsOut& h, uint32_t i, const
 
 // Traces one ray to completion.
 __device__ __forceinline__ void trace_full(const WorldView& w, RayState& st, uint32_t max_iters, bool wave_tame,
-                                           float& t, int4& rec) {
+                                           int origin, float& t, int4& rec) {
     int32_t v[3];
-    if (ray_trace_wave(w, st, max_iters, v, wave_tame))
+    if (ray_trace_wave(w, st, max_iters, v, wave_tame, origin))
         ray_hit_record(w, st, v, brick_slot_at(w, v), t, rec);
     else
         ray_miss_record(st, t, rec);
@@ -97,7 +97,7 @@ __global__ __launch_bounds__(kGridThreads) void cast_grid_kernel(WorldView w, Ra
     if (valid) {
         ray_setup(st, r.ox[i], r.oy[i], r.oz[i], r.dx[i], r.dy[i], r.dz[i], r.W[0], r.W[1], r.W[2]);
         // the fast loop when every ray of the wave is tame, else the exact one
-        trace_full(w, st, max_iters, __all(ray_is_tame(st)), t, rec);
+        trace_full(w, st, max_iters, __all(ray_is_tame(st)), origin_kind(r.W[0], r.W[1], r.W[2]), t, rec);
         store_result(h, i, st, t, rec);
     }
     if constexpr (COUNT) {
@@ -720,13 +720,29 @@ static void carve(void* base, uint32_t region_cap, ContSet& a, ContSet& b) {
 
 // ---------------------------------------------------------------- launch
 
+// Option checks that do not depend on the ray count, so a call is rejected or
+// accepted the same way whatever n is (n = 0 included).
+int check_cast_opts(const vrt_cast_opts& o) {
+    uint32_t sched = o.schedule == VRT_SCHED_AUTO ? VRT_SCHED_GRID : o.schedule;
+    if (sched > VRT_SCHED_LANES) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_cast_ex: unknown schedule");
+    if (o.order > VRT_ORDER_REVERSE) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_cast_ex: unknown order");
+    if (sched == VRT_SCHED_LANES && o.chunk > kLanesMaxRays)
+        return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_cast_ex: lanes schedule takes 1..8 rays per lane");
+    if (sched == VRT_SCHED_MULTIPASS && o.passes > 64)
+        return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_cast_ex: at most 64 passes");
+    if (sched == VRT_SCHED_PERSISTENT && (o.chunk % 64 != 0 || o.chunk > (1u << 20)))
+        return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_cast_ex: chunk must be a multiple of 64");
+    return VRT_OK;
+}
+
 int launch_cast(const vrt_world* world, const RaysIn& r, const HitsOut& h, uint32_t max_iters,
                 const vrt_cast_opts& o, hipStream_t s) {
     WorldView w = world->view();
+    int rc0 = check_cast_opts(o);
+    if (rc0 != VRT_OK) return rc0;
     if (r.n == 0) return VRT_OK;
     uint32_t sched = o.schedule == VRT_SCHED_AUTO ? VRT_SCHED_GRID : o.schedule;
     if (sched == VRT_SCHED_GRID) {
-        if (o.order > VRT_ORDER_REVERSE) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_cast_ex: unknown order");
         uint32_t blocks = (uint32_t)(((uint64_t)r.n + kGridThreads - 1) / kGridThreads);
         uint32_t rev = o.order == VRT_ORDER_REVERSE ? 1u : 0u;
         if (o.counters)
@@ -738,7 +754,6 @@ int launch_cast(const vrt_world* world, const RaysIn& r, const HitsOut& h, uint3
     }
     if (sched == VRT_SCHED_LANES) {
         uint32_t K = o.chunk ? o.chunk : 4;
-        if (K > kLanesMaxRays) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_cast_ex: lanes schedule takes 1..8 rays per lane");
         if (r.n > 0xFFFFF000u) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_cast_ex: too many rays");
         uint32_t blocks = (uint32_t)(((uint64_t)r.n + 64u * K - 1) / (64u * K));
         size_t lds = lanes_lds_bytes(K);
@@ -763,7 +778,6 @@ int launch_cast(const vrt_world* world, const RaysIn& r, const HitsOut& h, uint3
     if (sched == VRT_SCHED_MULTIPASS) {
         uint32_t K = o.pass_iters ? o.pass_iters : 16;
         uint32_t P = o.passes ? o.passes : 5;
-        if (P > 64) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_cast_ex: at most 64 passes");
         int cus = 0;
         int rc = device_info(world->device, &cus, nullptr);
         if (rc != VRT_OK) return rc;
@@ -810,9 +824,7 @@ int launch_cast(const vrt_world* world, const RaysIn& r, const HitsOut& h, uint3
         }
         return VRT_OK;
     }
-    if (sched != VRT_SCHED_PERSISTENT) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_cast_ex: unknown schedule");
     uint32_t chunk = o.chunk ? o.chunk : 64;
-    if (chunk % 64 != 0 || chunk > (1u << 20)) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_cast_ex: chunk must be a multiple of 64");
     if (r.n > 0x7FFFFFFFu) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_cast_ex: persistent schedule takes at most 2^31 - 1 rays");
     int cus = 0;
     uint32_t* queue = nullptr;

@@ -58,7 +58,8 @@ __global__ __launch_bounds__(64) void render_fwd_kernel(WorldView w, FwdViews vi
     }
     bool wave_tame = __all(tame);
     if (valid) {
-        bool hit = ray_trace_wave(w, st, max_iters, v, wave_tame);
+        bool hit = ray_trace_wave(w, st, max_iters, v, wave_tame,
+                                  origin_kind(cam.world_origin[0], cam.world_origin[1], cam.world_origin[2]));
         float wgt = 0.0f;
         uint32_t gv = 0xFFFFFFFFu;
         if (hit) {
@@ -150,7 +151,7 @@ int vrt_render_fwd(const vrt_world* world, const vrt_camera* cameras, uint32_t n
                    const float light_dir[3], float intensity, const float* albedo, float* image,
                    uint32_t* saved_voxel, float* saved_weight, void* stream) {
     return guarded([&]() -> int {
-        if (!world || (nviews && !cameras) || !light_dir || !albedo || !image || !saved_voxel || !saved_weight)
+        if (!world || (nviews && !cameras) || !light_dir)
             return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_render_fwd: null argument");
         if (max_iters == 0 || max_iters > 65535) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_render_fwd: max_iters must be 1..65535");
         if ((uint64_t)world->slot_cap * 512 >= 0xFFFFFFFFull)
@@ -165,6 +166,8 @@ int vrt_render_fwd(const vrt_world* world, const vrt_camera* cameras, uint32_t n
             total += (uint64_t)c.width * c.height;
         }
         if (total > 0xFFFFFFFFull) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_render_fwd: more than 2^32 - 1 pixels");
+        if (total && (!albedo || !image || !saved_voxel || !saved_weight))
+            return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_render_fwd: null argument");
         uint32_t base = 0;
         for (uint32_t k0 = 0; k0 < nviews; k0 += kFwdViews) {
             FwdViews v{};

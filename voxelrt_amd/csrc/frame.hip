@@ -96,10 +96,10 @@ __device__ __forceinline__ void flush4(vrt_cast_counters* ctr, unsigned long lon
 // Traces the wave's active rays, with the fast step when all of them are tame;
 // true on a hit, with v the hit voxel.
 __device__ __forceinline__ bool trace_wave(const WorldView& w, RayState& st, bool active, uint32_t max_iters,
-                                           int32_t v[3]) {
+                                           int origin, int32_t v[3]) {
     bool tame = __all(!active || ray_is_tame(st));
     if (!active) return false;
-    return ray_trace_wave(w, st, max_iters, v, tame);
+    return ray_trace_wave(w, st, max_iters, v, tame, origin);
 }
 
 __device__ __forceinline__ void finish_trace(const WorldView& w, const RayState& st, bool hit, const int32_t v[3],
@@ -121,7 +121,8 @@ __global__ __launch_bounds__(256) void frame_grid_kernel(WorldView w, FrameArgs 
     RayState st;
     int32_t v[3] = {0, 0, 0};
     if (valid) setup_primary(a, i, st);
-    bool hit = trace_wave(w, st, valid, a.max_iters, v);
+    const int origin = origin_kind(a.cam.world_origin[0], a.cam.world_origin[1], a.cam.world_origin[2]);
+    bool hit = trace_wave(w, st, valid, a.max_iters, origin, v);
     bool bounce = false;
     uint32_t info = 0;
     if (valid) {
@@ -134,7 +135,7 @@ __global__ __launch_bounds__(256) void frame_grid_kernel(WorldView w, FrameArgs 
         bounce = (info & 0xFF) != 0;
         if (bounce) setup_bounce(a, i, info, st);
     }
-    hit = trace_wave(w, st, bounce, a.max_iters, v);
+    hit = trace_wave(w, st, bounce, a.max_iters, origin, v);
     if (valid) {
         if (bounce) {
             float t;

@@ -25,8 +25,22 @@ std::vector<vrt_camera> unpack_cameras(const torch::Tensor& cams) {
                 "cameras: contiguous uint8 CPU tensor of packed vrt_camera structs");
     TORCH_CHECK(cams.numel() % sizeof(vrt_camera) == 0, "cameras: size is not a multiple of sizeof(vrt_camera)");
     std::vector<vrt_camera> out(cams.numel() / sizeof(vrt_camera));
-    std::memcpy(out.data(), cams.data_ptr(), cams.numel());
+    if (!out.empty()) std::memcpy(out.data(), cams.data_ptr(), cams.numel());
     return out;
+}
+
+// The kernels index albedo by slot * 512 + voxel for every allocated slot, so it must
+// cover the world's slot high-water mark (the C ABI cannot check a raw pointer's length).
+void check_inputs(const torch::Tensor& albedo, int64_t world, const std::vector<double>& light) {
+    TORCH_CHECK(albedo.is_cuda() && albedo.scalar_type() == torch::kFloat32 && albedo.dim() == 2 &&
+                    albedo.size(1) == 3,
+                "albedo: float32 [voxels, 3] on the GPU");
+    TORCH_CHECK(light.size() == 3, "light: 3 components");
+    vrt_world_info info;
+    check(vrt_world_get_info((const vrt_world*)world, &info), "vrt_world_get_info");
+    TORCH_CHECK(albedo.size(0) >= (int64_t)info.slot_high_water * 512,
+                "albedo: needs slot_high_water * 512 = ", (int64_t)info.slot_high_water * 512, " voxels, has ",
+                albedo.size(0));
 }
 
 class RenderFunction : public torch::autograd::Function<RenderFunction> {
@@ -34,10 +48,7 @@ public:
     static torch::Tensor forward(torch::autograd::AutogradContext* ctx, torch::Tensor albedo, int64_t world,
                                  torch::Tensor cameras, int64_t max_iters, std::vector<double> light,
                                  double intensity, int64_t stream) {
-        TORCH_CHECK(albedo.is_cuda() && albedo.scalar_type() == torch::kFloat32 && albedo.dim() == 2 &&
-                        albedo.size(1) == 3,
-                    "albedo: float32 [voxels, 3] on the GPU");
-        TORCH_CHECK(light.size() == 3, "light: 3 components");
+        check_inputs(albedo, world, light);
         auto cams = unpack_cameras(cameras);
         int64_t npix = 0;
         for (const auto& c : cams) npix += (int64_t)c.width * c.height;
@@ -84,6 +95,7 @@ torch::Tensor render(torch::Tensor albedo, int64_t world, torch::Tensor cameras,
 // Forward only, also returning the saved hit voxel ids and shading weights.
 std::vector<torch::Tensor> render_saved(torch::Tensor albedo, int64_t world, torch::Tensor cameras, int64_t max_iters,
                                         std::vector<double> light, double intensity, int64_t stream) {
+    check_inputs(albedo, world, light);
     auto cams = unpack_cameras(cameras);
     int64_t npix = 0;
     for (const auto& c : cams) npix += (int64_t)c.width * c.height;

@@ -20,6 +20,16 @@
 //                  takes its INT_MIN branch and the two minimums agree up to
 //                  the sign of a zero, which cannot change tmin + 0.001.
 // The reported t is recomputed with the exact x86 minimum in both cases.
+//
+// A third template parameter, ORIGIN, specializes the step for the rays' integer
+// world origin W (uniform over a launch or a batch segment):
+//   ORIGIN_ANY      any W: voxel = W + floor(cur), rel = aligned voxel - W;
+//   ORIGIN_ALIGNED  W a multiple of 32 on every axis (a sector corner): the
+//                   largest cell a step aligns to is 32 voxels (mask 31), so
+//                   aligning W + f and subtracting W gives the same bits as
+//                   aligning f itself; the step skips the three subtractions;
+//   ORIGIN_ZERO     W = 0: voxel = floor(cur), so the additions go as well.
+// Every form computes the same bits as ORIGIN_ANY.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -130,13 +140,22 @@ __device__ __forceinline__ int32_t floor_i32_in_range(float x) {
 //     is cm = ce >> min(group bits, 1); the sector record holds ce for its mask;
 //   alignment: (v & ~cm) | (dm & cm), one bitfield insert per axis.
 // Occupancy indices slot * 8 + cell are 32-bit: worlds hold at most 2^28 slots.
-template <bool EXACT, bool CUBIC = false>
+enum OriginKind : int { ORIGIN_ANY = 0, ORIGIN_ALIGNED = 1, ORIGIN_ZERO = 2 };
+
+// The specialization that a world origin allows (uniform over the caller's rays).
+__device__ __forceinline__ int origin_kind(int32_t wx, int32_t wy, int32_t wz) {
+    if ((wx | wy | wz) == 0) return ORIGIN_ZERO;
+    return ((wx | wy | wz) & 31) == 0 ? ORIGIN_ALIGNED : ORIGIN_ANY;
+}
+
+template <bool EXACT, bool CUBIC = false, int ORIGIN = ORIGIN_ANY>
 __device__ __forceinline__ bool ray_advance(const WorldView& w, RayState& r, int32_t v[3], bool& hit,
                                             uint32_t* hit_slot) {
+    int32_t f[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-        int32_t f = EXACT ? floor2i(r.cur[a]) : floor_i32_in_range(r.cur[a]);
-        v[a] = (int32_t)((uint32_t)r.W[a] + (uint32_t)f);
+        f[a] = EXACT ? floor2i(r.cur[a]) : floor_i32_in_range(r.cur[a]);
+        v[a] = ORIGIN == ORIGIN_ZERO ? f[a] : (int32_t)((uint32_t)r.W[a] + (uint32_t)f[a]);
     }
     // GetInboundMask (:114-117); one compare when the world is a cube
     bool inb = CUBIC ? (uint32_t)(v[0] | v[1] | v[2]) < w.size_xz
@@ -188,9 +207,14 @@ __device__ __forceinline__ bool ray_advance(const WorldView& w, RayState& r, int
     uint32_t cm = ce >> min_1(grp);
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-        // d < 0: v & ~cm (near corner); d >= 0: v | cm (far corner)
-        v[a] = (int32_t)bitfield_insert(~cm, (uint32_t)v[a], (uint32_t)r.dm[a]);
-        int32_t rel = (int32_t)((uint32_t)v[a] - (uint32_t)r.W[a]);
+        // d < 0: v & ~cm (near corner); d >= 0: v | cm (far corner); rel = that corner - W
+        int32_t rel;
+        if (ORIGIN != ORIGIN_ANY) {
+            rel = (int32_t)bitfield_insert(~cm, (uint32_t)f[a], (uint32_t)r.dm[a]);
+        } else {
+            v[a] = (int32_t)bitfield_insert(~cm, (uint32_t)v[a], (uint32_t)r.dm[a]);
+            rel = (int32_t)((uint32_t)v[a] - (uint32_t)r.W[a]);
+        }
         r.side[a] = fmaf((float)rel, r.inv[a], r.ts[a]);
     }
     float tmin;
@@ -228,22 +252,32 @@ __device__ __forceinline__ uint32_t brick_slot_at(const WorldView& w, const int3
 // RayCast's loop (CpuRenderer.cpp:185-202): steps until a hit, a miss or
 // max_iters steps; true on a hit, with v the hit voxel (its slot is
 // brick_slot_at(w, v)).
-template <bool EXACT, bool CUBIC = false>
+template <bool EXACT, bool CUBIC = false, int ORIGIN = ORIGIN_ANY>
 __device__ __forceinline__ bool ray_trace(const WorldView& w, RayState& r, uint32_t max_iters, int32_t v[3]) {
     bool hit = false;
     uint32_t unused;
     for (uint32_t it = 0; it < max_iters; it++)
-        if (!ray_advance<EXACT, CUBIC>(w, r, v, hit, &unused)) break;
+        if (!ray_advance<EXACT, CUBIC, ORIGIN>(w, r, v, hit, &unused)) break;
     return hit;
 }
 
+template <bool CUBIC>
+__device__ __forceinline__ bool ray_trace_tame(const WorldView& w, RayState& r, uint32_t max_iters, int32_t v[3],
+                                               int origin) {
+    if (origin == ORIGIN_ZERO) return ray_trace<false, CUBIC, ORIGIN_ZERO>(w, r, max_iters, v);
+    if (origin == ORIGIN_ALIGNED) return ray_trace<false, CUBIC, ORIGIN_ALIGNED>(w, r, max_iters, v);
+    return ray_trace<false, CUBIC>(w, r, max_iters, v);
+}
+
 // ray_trace with the step picked per wave: the fast step when every ray of the
-// wave is tame (wave_tame), with the single-compare bounds test in cubic worlds.
+// wave is tame (wave_tame), with the single-compare bounds test in cubic worlds
+// and specialized for the rays' world origin (origin: an OriginKind, uniform over
+// the wave).
 __device__ __forceinline__ bool ray_trace_wave(const WorldView& w, RayState& r, uint32_t max_iters, int32_t v[3],
-                                               bool wave_tame) {
+                                               bool wave_tame, int origin = ORIGIN_ANY) {
     if (!wave_tame) return ray_trace<true>(w, r, max_iters, v);
-    if (w.size_y == w.size_xz) return ray_trace<false, true>(w, r, max_iters, v);
-    return ray_trace<false>(w, r, max_iters, v);
+    if (w.size_y == w.size_xz) return ray_trace_tame<true>(w, r, max_iters, v, origin);
+    return ray_trace_tame<false>(w, r, max_iters, v, origin);
 }
 
 // Hit record fields (vrt_hit in vrt.h) of a finished ray.

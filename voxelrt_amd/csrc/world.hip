@@ -301,7 +301,9 @@ int vrt_world_get_info(const vrt_world* w, vrt_world_info* out) {
         out->num_sectors = w->nsec;
         out->num_bricks = w->num_bricks;
         out->slot_capacity = w->slot_cap;
-        out->device_bytes = (uint64_t)w->nsec * 16 + 256 * 8 + (uint64_t)w->slot_cap * (512 + 64);
+        out->device_bytes = (uint64_t)w->nsec * 16 + 256 * 8 + (uint64_t)w->slot_cap * (512 + 64) +
+                            (uint64_t)w->h_groups.size() * 8 + 512 * 8;
+        out->slot_high_water = w->arena.high_water();
         return VRT_OK;
     });
 }

@@ -7,6 +7,10 @@
         sparse set of 587 spheres (~2% fill); 1920x1080 primary + 2M incoherent rays
     C4  the C2 world, 64 cameras of uneven square sizes + bounce sets (batched)
     C5  512^3 terrain, 16 views of 512x512, per-voxel RGB albedo (differentiable)
+
+The C2, C4 and C5 cameras put their rays' integer world origin at the corner of the
+camera's sector (a multiple of 32 per axis: the casts' aligned-origin step); C1 and C3
+at floor(eye), the reference's choice (the general step).
 """
 from __future__ import annotations
 
@@ -56,8 +60,9 @@ def c1_camera(tile=(8, 8), width=640, height=360) -> CameraSpec:
     return CameraSpec((128.37, 200.61, 12.29), yaw_deg=8.0, pitch_deg=-28.0, width=width, height=height, tile=tile)
 
 
-def c2_camera(tile=(8, 8), width=1920, height=1080) -> CameraSpec:
-    return CameraSpec((512.37, 420.61, 96.29), yaw_deg=12.0, pitch_deg=-24.0, width=width, height=height, tile=tile)
+def c2_camera(tile=(8, 8), width=1920, height=1080, anchor="sector") -> CameraSpec:
+    return CameraSpec((512.37, 420.61, 96.29), yaw_deg=12.0, pitch_deg=-24.0, width=width, height=height, tile=tile,
+                      anchor=anchor)
 
 
 def c3_world(sparse: bool, threads: int = 0) -> Scene:
@@ -80,7 +85,7 @@ def c4_cameras(n: int = 64, seed: int = 4, tile=(8, 8)) -> list[CameraSpec]:
         pos = (float(rng.uniform(64, 960)), float(rng.uniform(350, 500)), float(rng.uniform(64, 960)))
         yaw = float(rng.uniform(0, 360))
         pitch = float(rng.uniform(-50, -10))
-        cams.append(CameraSpec(pos, yaw, pitch, size, size, tile=tile))
+        cams.append(CameraSpec(pos, yaw, pitch, size, size, tile=tile, anchor="sector"))
     return cams
 
 
@@ -90,7 +95,7 @@ def c5_cameras(n: int = 16, size: int = 512, tile=(8, 8)) -> list[CameraSpec]:
         a = 2 * math.pi * k / n
         pos = (256 + 200 * math.sin(a) + 0.31, 420.57, 256 + 200 * math.cos(a) + 0.17)
         yaw = math.degrees(a) + 180.0
-        cams.append(CameraSpec(pos, yaw, -35.0, size, size, tile=tile))
+        cams.append(CameraSpec(pos, yaw, -35.0, size, size, tile=tile, anchor="sector"))
     return cams
 
 
