@@ -164,9 +164,13 @@ static int grow_slots(vrt_world* w, uint32_t need, hipStream_t s) {
         return hip_fail(e, "hipMalloc(voxels)");
     }
     if (w->slot_cap) {
-        VRT_HIP(hipMemcpyAsync(occ, w->d_occ, (size_t)w->slot_cap * 64, hipMemcpyDeviceToDevice, s));
-        VRT_HIP(hipMemcpyAsync(vox, w->d_vox, (size_t)w->slot_cap * 512, hipMemcpyDeviceToDevice, s));
-        VRT_HIP(hipStreamSynchronize(s));
+        if ((e = hipMemcpyAsync(occ, w->d_occ, (size_t)w->slot_cap * 64, hipMemcpyDeviceToDevice, s)) != hipSuccess ||
+            (e = hipMemcpyAsync(vox, w->d_vox, (size_t)w->slot_cap * 512, hipMemcpyDeviceToDevice, s)) != hipSuccess ||
+            (e = hipStreamSynchronize(s)) != hipSuccess) {
+            (void)hipFree(occ);  // the world keeps its old buffers
+            (void)hipFree(vox);
+            return hip_fail(e, "growing the slot buffers");
+        }
     }
     (void)hipFree(w->d_occ);
     (void)hipFree(w->d_vox);
@@ -231,8 +235,10 @@ int vrt_world_create(const vrt_world_desc* desc, void* stream, vrt_world** out) 
         if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
             return fail(VRT_ERR_NO_DEVICE, "vrt_world_create: no HIP device");
         hipStream_t s = (hipStream_t)stream;
+        int dev = 0;
+        VRT_HIP(hipGetDevice(&dev));
         auto* w = new vrt_world();
-        VRT_HIP(hipGetDevice(&w->device));
+        w->device = dev;
         w->log2_xz = desc->log2_size_xz;
         w->log2_y = desc->log2_size_y;
         w->shift_xz = w->log2_xz - 5;
