@@ -240,25 +240,6 @@ constexpr uint32_t kLanesMaxRays = 8;
 constexpr uint32_t kLaneWords = 15;
 __host__ __device__ constexpr uint32_t lanes_lds_bytes(uint32_t k) { return k * kLaneWords * 64 * 4; }
 
-// Whether voxel v (any coordinates) is solid, with its brick slot when it is.  A
-// parked ray needs no hit flag: a hit leaves v on the solid voxel, a miss outside
-// the world, and the cap on a corner of the empty cell it was stepping over.
-__device__ __forceinline__ bool solid_voxel_at(const WorldView& w, const int32_t v[3], uint32_t& slot) {
-    uint32_t ux = (uint32_t)v[0], uy = (uint32_t)v[1], uz = (uint32_t)v[2];
-    if (!((ux | uz) < w.size_xz && uy < w.size_y)) return false;
-    uint32_t si = (ux >> 5) | (uz >> 5) << w.shift_xz | (uy >> 5) << (2 * w.shift_xz);
-    uint4 sec = w.sec[si];
-    uint64_t m = (uint64_t)sec.x | (uint64_t)sec.y << 32;
-    uint32_t bi = __builtin_amdgcn_ubfe(ux, 3, 2) | __builtin_amdgcn_ubfe(uz, 3, 2) << 2 |
-                  __builtin_amdgcn_ubfe(uy, 3, 2) << 4;
-    if (!((m >> bi) & 1ull)) return false;
-    slot = sec.z + (uint32_t)__popcll(m & ((1ull << bi) - 1ull));
-    uint32_t cell = __builtin_amdgcn_ubfe(ux, 2, 1) | __builtin_amdgcn_ubfe(uz, 2, 1) << 1 |
-                    __builtin_amdgcn_ubfe(uy, 2, 1) << 2;
-    uint32_t vi = (ux & 3u) | (uz & 3u) << 2 | (uy & 3u) << 4;
-    return (w.occ[(size_t)slot * 8u + cell] >> vi) & 1ull;
-}
-
 // Row offsets (in words of the lane's column) of a ray's LDS slot.
 enum : uint32_t {
     kLaneO = 0, kLaneD = 3 * 64, kLaneInv = 6 * 64, kLaneTs = 9 * 64, kLaneDm = 12 * 64,  // waiting

@@ -165,6 +165,9 @@ __device__ __forceinline__ bool ray_advance(const WorldView& w, RayState& r, int
         return false;
     }
     r.iters++;
+    // a per-lane count: left to itself the compiler derives it from a wave-uniform loop
+    // counter with three copies per step
+    asm volatile("" : "+v"(r.iters));
     uint32_t ux = (uint32_t)v[0], uy = (uint32_t)v[1], uz = (uint32_t)v[2];
     uint32_t si = lshl_or(uy >> 5, 2 * w.shift_xz, lshl_or(uz >> 5, w.shift_xz, ux >> 5));
     // one 16-byte load for the whole record, held as one register quad: a descent then
@@ -249,6 +252,25 @@ __device__ __forceinline__ uint32_t brick_slot_at(const WorldView& w, const int3
     return sec.z + (uint32_t)__popcll(m & ((1ull << bi) - 1ull));
 }
 
+// Whether voxel v (any coordinates) is solid, with its brick slot when it is.  A
+// parked ray needs no hit flag: a hit leaves v on the solid voxel, a miss outside
+// the world, and the cap on a corner of the empty cell it was stepping over.
+__device__ __forceinline__ bool solid_voxel_at(const WorldView& w, const int32_t v[3], uint32_t& slot) {
+    uint32_t ux = (uint32_t)v[0], uy = (uint32_t)v[1], uz = (uint32_t)v[2];
+    if (!((ux | uz) < w.size_xz && uy < w.size_y)) return false;
+    uint32_t si = (ux >> 5) | (uz >> 5) << w.shift_xz | (uy >> 5) << (2 * w.shift_xz);
+    uint4 sec = w.sec[si];
+    uint64_t m = (uint64_t)sec.x | (uint64_t)sec.y << 32;
+    uint32_t bi = __builtin_amdgcn_ubfe(ux, 3, 2) | __builtin_amdgcn_ubfe(uz, 3, 2) << 2 |
+                  __builtin_amdgcn_ubfe(uy, 3, 2) << 4;
+    if (!((m >> bi) & 1ull)) return false;
+    slot = sec.z + (uint32_t)__popcll(m & ((1ull << bi) - 1ull));
+    uint32_t cell = __builtin_amdgcn_ubfe(ux, 2, 1) | __builtin_amdgcn_ubfe(uz, 2, 1) << 1 |
+                    __builtin_amdgcn_ubfe(uy, 2, 1) << 2;
+    uint32_t vi = (ux & 3u) | (uz & 3u) << 2 | (uy & 3u) << 4;
+    return (w.occ[(size_t)slot * 8u + cell] >> vi) & 1ull;
+}
+
 // RayCast's loop (CpuRenderer.cpp:185-202): steps until a hit, a miss or
 // max_iters steps; true on a hit, with v the hit voxel (its slot is
 // brick_slot_at(w, v)).
@@ -256,9 +278,20 @@ template <bool EXACT, bool CUBIC = false, int ORIGIN = ORIGIN_ANY>
 __device__ __forceinline__ bool ray_trace(const WorldView& w, RayState& r, uint32_t max_iters, int32_t v[3]) {
     bool hit = false;
     uint32_t unused;
+    if (EXACT) {
+        for (uint32_t it = 0; it < max_iters; it++)
+            if (!ray_advance<EXACT, CUBIC, ORIGIN>(w, r, v, hit, &unused)) break;
+        return hit;
+    }
+    // The fast loop carries no hit flag (its per-iteration lane-mask bookkeeping costs
+    // scalar instructions, and one scalar unit serves four SIMDs): the exit is told
+    // apart after the loop, as a hit leaves v on the solid voxel, a miss outside the
+    // world, and the cap on a voxel of the empty cell it was stepping over (every entry
+    // point rejects a cap of 0, so v is set by at least one step).
     for (uint32_t it = 0; it < max_iters; it++)
         if (!ray_advance<EXACT, CUBIC, ORIGIN>(w, r, v, hit, &unused)) break;
-    return hit;
+    uint32_t slot;
+    return solid_voxel_at(w, v, slot);
 }
 
 template <bool CUBIC>
