@@ -132,3 +132,37 @@ def test_bounce_compaction_of_many_blocks(vrt, cuda):
     assert cnt.value == nb and nb > 1_000_000
     assert np.array_equal(src[:nb].cpu().numpy().astype(np.uint32), hsrc[:nb])
     assert np.array_equal(out[3:, :nb].view(np.uint32), br.d[:, :nb].cpu().numpy().view(np.uint32))
+
+
+@pytest.mark.parametrize("n,frac", [(1, 1.0), (4095, 0.5), (4097, 0.3), (1_000_003, 0.45), (8_388_608, 0.5),
+                                    (8_388_609, 0.02), (9_000_000, 0.6)])
+def test_bounce_generation_paths_match_host_twin(vrt, cuda, n, frac):
+    """Synthetic hit sets on both sides of the two-launch generation's limit (2048 blocks of
+    4096 primaries; above it count -> scan -> scatter): rays, sources and count equal the
+    host twin's, with ragged last blocks and sparse and dense hits."""
+    import voxelrt_amd as V
+    from voxelrt_amd import _lib as L
+
+    rng = np.random.default_rng(n)
+    rec = np.zeros((n, 4), np.int32)
+    hit = rng.random(n) < frac
+    codes = rng.integers(0, 6, n)  # one axis, either sign: the 2-bit normal fields at bits 8/10/12
+    info = np.where(hit, 1 + rng.integers(0, 255, n), 0)
+    field = (np.where(codes % 2 == 0, 1, 3) << (8 + 2 * (codes // 2)))
+    rec[:, 3] = np.where(hit, info | field, 0).astype(np.int32)
+    pos = (rng.random((3, n), dtype=np.float32) * 64.0).astype(np.float32)
+    hits = V.HitBatch(torch.zeros(n, device=cuda), torch.from_numpy(rec).to(cuda), torch.from_numpy(pos).to(cuda))
+    prim = V.RayBatch.empty(n, (3, -5, 7))
+    br, src, count = V.bounce(prim, hits, 1920, 7)
+    torch.cuda.synchronize()
+    nb = int(count.item())
+    out = np.zeros((6, n), np.float32)
+    hsrc = np.zeros(n, np.uint32)
+    cnt = C.c_uint32()
+    ptrs = (C.c_void_p * 6)(*[out[i].ctypes.data for i in range(6)])
+    L.check(V.lib().vrt_bounce_host(n, rec.ctypes.data, pos.ctypes.data, 1920, 7, C.byref(ptrs), hsrc.ctypes.data,
+                                    C.byref(cnt)))
+    assert cnt.value == nb == int(hit.sum())
+    assert np.array_equal(src[:nb].cpu().numpy().astype(np.uint32), hsrc[:nb])
+    assert np.array_equal(out[:3, :nb].view(np.uint32), br.o[:, :nb].cpu().numpy().view(np.uint32))
+    assert np.array_equal(out[3:, :nb].view(np.uint32), br.d[:, :nb].cpu().numpy().view(np.uint32))

@@ -5,7 +5,8 @@
 // vrt_bounce_generate replaces the bounce construction of RenderRow
 // (CpuRenderer.cpp:389-392) / VoxelRender.comp:63-71, with the PCG sample of
 // RandomGen.glsl:6-20, and compacts the bounce rays in primary order
-// (count -> scan -> scatter; deterministic, no atomics).
+// (count -> scatter, each scatter block summing the counts before it; beyond 8.4 M
+// primaries count -> scan -> scatter; deterministic, no atomics).
 #include <hip/hip_runtime.h>
 
 #include <cstring>
@@ -133,6 +134,102 @@ __global__ __launch_bounds__(256) void bounce_write_kernel(const int4* __restric
     src[k] = i;
 }
 
+// Two-launch generation for up to kGenMaxBlocks blocks of kGenBlock primaries (8.4 M):
+// the count launch writes one hit count per block; each write block then sums the counts
+// of the blocks before it (at most two loads per thread) instead of waiting for a scan
+// launch, and block 0 also writes the total.  Thread t of block b handles primaries
+// b * kGenBlock + j * 1024 + t, j = 0..3, so the output order is the primary order.
+static const uint32_t kGenRows = 4;
+static const uint32_t kGenBlock = 1024 * kGenRows;
+static const uint32_t kGenMaxBlocks = 2048;
+
+__global__ __launch_bounds__(1024) void bounce_count_rows_kernel(const int4* __restrict__ rec, uint32_t n,
+                                                                 uint32_t* __restrict__ block_counts) {
+    __shared__ uint32_t wave_counts[16];
+    const uint32_t t = threadIdx.x;
+    uint32_t c = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < kGenRows; j++) {
+        uint32_t i = blockIdx.x * kGenBlock + j * 1024 + t;
+        bool hit = i < n && (rec[i].w & 0xFF) != 0;
+        c += (uint32_t)__popcll(__ballot(hit));
+    }
+    if (__lane_id() == 0) wave_counts[t >> 6] = c;
+    __syncthreads();
+    if (t < 64) {
+        uint32_t v = t < 16 ? wave_counts[t] : 0u;
+#pragma unroll
+        for (int off = 8; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        if (t == 0) block_counts[blockIdx.x] = v;
+    }
+}
+
+__global__ __launch_bounds__(1024) void bounce_write_rows_kernel(const int4* __restrict__ rec,
+                                                                 const float* __restrict__ px,
+                                                                 const float* __restrict__ py,
+                                                                 const float* __restrict__ pz, uint32_t n,
+                                                                 uint32_t width, uint32_t seed,
+                                                                 const uint32_t* __restrict__ block_counts,
+                                                                 uint32_t nblocks, vrt_ray_buffers out,
+                                                                 uint32_t* __restrict__ src,
+                                                                 uint32_t* __restrict__ count) {
+    __shared__ uint32_t wave_sums[16];
+    __shared__ uint32_t row_base[kGenRows * 16];  // per (row, wave): hits before it in this block
+    const uint32_t t = threadIdx.x, lane = __lane_id(), wave = t >> 6, b = blockIdx.x;
+    // hits of the blocks before this one (block 0: of all blocks, the total)
+    const uint32_t lim = b == 0 ? nblocks : b;
+    uint32_t before = 0;
+    for (uint32_t k = t; k < lim; k += 1024) before += block_counts[k];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) before += __shfl_xor(before, off, 64);
+    if (lane == 0) wave_sums[wave] = before;
+    int4 r[kGenRows];
+    uint64_t ball[kGenRows];
+#pragma unroll
+    for (uint32_t j = 0; j < kGenRows; j++) {
+        uint32_t i = b * kGenBlock + j * 1024 + t;
+        r[j] = i < n ? rec[i] : make_int4(0, 0, 0, 0);
+        ball[j] = __ballot(i < n && (r[j].w & 0xFF) != 0);
+        if (lane == 0) row_base[j * 16 + wave] = (uint32_t)__popcll(ball[j]);
+    }
+    __syncthreads();
+    if (t < 64) {
+        // exclusive scan of the 64 (row, wave) counts in row-major order = primary order
+        uint32_t c = row_base[t], incl = c;
+#pragma unroll
+        for (uint32_t off = 1; off < 64; off <<= 1) {
+            uint32_t y = __shfl_up(incl, off, 64);
+            if (t >= off) incl += y;
+        }
+        uint32_t base = 0;
+#pragma unroll
+        for (uint32_t w = 0; w < 16; w++) base += wave_sums[w];
+        if (b == 0) {
+            if (t == 0) *count = base;  // block 0 summed every block: the total
+            base = 0;
+        }
+        row_base[t] = base + incl - c;
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t j = 0; j < kGenRows; j++) {
+        uint32_t i = b * kGenBlock + j * 1024 + t;
+        if (!(i < n && (r[j].w & 0xFF) != 0)) continue;
+        uint32_t k = row_base[j * 16 + wave] +
+                     __builtin_amdgcn_mbcnt_hi((uint32_t)(ball[j] >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)ball[j], 0u));
+        float p[3] = {px[i], py[i], pz[i]};
+        float o[3], d[3];
+        bounce_ray((uint32_t)r[j].w, p, i, width, seed, o, d);
+        out.ox[k] = o[0];
+        out.oy[k] = o[1];
+        out.oz[k] = o[2];
+        out.dx[k] = d[0];
+        out.dy[k] = d[1];
+        out.dz[k] = d[2];
+        src[k] = i;
+    }
+}
+
 }  // namespace vrt
 
 using namespace vrt;
@@ -181,6 +278,14 @@ int vrt_bounce_generate(const vrt_rays* primary, const vrt_hits* hits, uint32_t 
             return VRT_OK;
         }
         const int4* rec = (const int4*)hits->rec;
+        const uint32_t nrows = (n + kGenBlock - 1) / kGenBlock;
+        if (nrows <= kGenMaxBlocks) {
+            bounce_count_rows_kernel<<<nrows, 1024, 0, s>>>(rec, n, blocks);
+            bounce_write_rows_kernel<<<nrows, 1024, 0, s>>>(rec, hits->pos, hits->pos + n, hits->pos + 2 * (size_t)n,
+                                                            n, width, seed, blocks, nrows, *out, src, count);
+            VRT_HIP(hipGetLastError());
+            return VRT_OK;
+        }
         bounce_count_kernel<<<nb, 256, 0, s>>>(rec, n, blocks);
         bounce_scan_kernel<<<1, 1024, 0, s>>>(blocks, nb, count);
         bounce_write_kernel<<<nb, 256, 0, s>>>(rec, hits->pos, hits->pos + n, hits->pos + 2 * (size_t)n, n, width,
