@@ -30,6 +30,14 @@
 //                   aligning f itself; the step skips the three subtractions;
 //   ORIGIN_ZERO     W = 0: voxel = floor(cur), so the additions go as well.
 // Every form computes the same bits as ORIGIN_ANY.
+//
+// ONE_EXIT (the fast loop): the lookup is made before the bounds test, so a step
+// leaves the loop at one place for both ends of a ray.  A lane that has left the
+// world reads a sector index past the records, which the structured buffer load
+// answers with 0 (an empty sector), and it does not descend.  The loop keeps no hit
+// flag: ray_trace tells the exits apart with one lookup after the loop.  Each exit
+// and flag costs lane-mask bookkeeping on the scalar unit, which one CU's four SIMDs
+// share, so the one-exit step issues 25 scalar instructions against 36.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -81,6 +89,25 @@ __device__ __forceinline__ bool ray_is_tame(const RayState& r) {
 // (mask & a) | (~mask & b): V_BFI_B32.
 __device__ __forceinline__ uint32_t bitfield_insert(uint32_t mask, uint32_t a, uint32_t b) {
     return (mask & a) | (~mask & b);
+}
+
+// Structured buffer load (BUFFER_LOAD_DWORDX4 ... IDXEN): record vindex of the
+// resource's array, the byte address index * stride computed by the load itself, and
+// 0 for an index at or past the resource's record count (range-checked in hardware).
+typedef int32_t i32x4 __attribute__((ext_vector_type(4)));
+__device__ i32x4 vrt_struct_buffer_load_v4i32(i32x4 rsrc, int32_t vindex, int32_t voffset, int32_t soffset,
+                                              int32_t aux) __asm("llvm.amdgcn.struct.buffer.load.v4i32");
+
+// Buffer resource of the sector records: base address, 16-byte stride, one record per
+// sector.  Word 3 is the gfx9 untyped-buffer configuration.
+__device__ __forceinline__ i32x4 sector_records_rsrc(const WorldView& w) {
+    uint64_t a = (uint64_t)w.sec;
+    i32x4 r;
+    r.x = (int32_t)(uint32_t)a;
+    r.y = (int32_t)(((uint32_t)(a >> 32) & 0xFFFFu) | (16u << 16));
+    r.z = (int32_t)(1u << (2 * w.shift_xz + w.shift_y));
+    r.w = 0x00020000;
+    return r;
 }
 
 // (a << sh) | b in one V_LSHL_OR_B32.
@@ -148,7 +175,7 @@ __device__ __forceinline__ int origin_kind(int32_t wx, int32_t wy, int32_t wz) {
     return ((wx | wy | wz) & 31) == 0 ? ORIGIN_ALIGNED : ORIGIN_ANY;
 }
 
-template <bool EXACT, bool CUBIC = false, int ORIGIN = ORIGIN_ANY>
+template <bool EXACT, bool CUBIC = false, int ORIGIN = ORIGIN_ANY, bool ONE_EXIT = false>
 __device__ __forceinline__ bool ray_advance(const WorldView& w, RayState& r, int32_t v[3], bool& hit,
                                             uint32_t* hit_slot) {
     int32_t f[3];
@@ -160,20 +187,24 @@ __device__ __forceinline__ bool ray_advance(const WorldView& w, RayState& r, int
     // GetInboundMask (:114-117); one compare when the world is a cube
     bool inb = CUBIC ? (uint32_t)(v[0] | v[1] | v[2]) < w.size_xz
                      : ((uint32_t)(v[0] | v[2]) < w.size_xz && (uint32_t)v[1] < w.size_y);
-    if (!inb) {
-        hit = false;
-        return false;
+    if (!ONE_EXIT) {
+        if (!inb) {
+            hit = false;
+            return false;
+        }
+        r.iters++;
+        // a per-lane count: left to itself the compiler derives it from a wave-uniform loop
+        // counter with three copies per step
+        asm volatile("" : "+v"(r.iters));
     }
-    r.iters++;
-    // a per-lane count: left to itself the compiler derives it from a wave-uniform loop
-    // counter with three copies per step
-    asm volatile("" : "+v"(r.iters));
     uint32_t ux = (uint32_t)v[0], uy = (uint32_t)v[1], uz = (uint32_t)v[2];
     uint32_t si = lshl_or(uy >> 5, 2 * w.shift_xz, lshl_or(uz >> 5, w.shift_xz, ux >> 5));
     // one 16-byte load for the whole record, held as one register quad: a descent then
     // waits on one load (the occupancy mask) instead of two
+    // (a structured buffer load: the hardware scales the index, one instruction fewer
+    // than a 64-bit address)
     typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-    u32x4 sec = *(const u32x4*)&w.sec[si];
+    u32x4 sec = (u32x4)vrt_struct_buffer_load_v4i32(sector_records_rsrc(w), (int32_t)si, 0, 0, 0);
     asm volatile("" : "+v"(sec));
     uint64_t m = (uint64_t)sec.x | (uint64_t)sec.y << 32;
     // brick within the sector: bits 3..4 of x, z, y at bits 0..1, 2..3, 4..5
@@ -181,6 +212,7 @@ __device__ __forceinline__ bool ray_advance(const WorldView& w, RayState& r, int
     uint32_t nbi = insert_bits_not<0x30u>(uy << 1, insert_bits<0x0Cu>(uz >> 1, ux >> 3));
     uint64_t below = m << (nbi & 63u);
     bool descend = (int32_t)(below >> 32) < 0;
+    if (ONE_EXIT) descend = descend && inb;
     // the cell mask of a step whose 2x2x2 group is empty: 15 below a sector mask, 31
     // when that mask is 0 (from the record); 1 below a voxel mask, 3 when it is 0
     uint32_t ce = sec.w;
@@ -198,8 +230,13 @@ __device__ __forceinline__ bool ray_advance(const WorldView& w, RayState& r, int
         ce = m == 0 ? 3u : 1u;
     }
     // the hit test after the join: its complement is then a scalar mask operation
-    // rather than a second vector compare of the sector mask
-    if (solid) {
+    // rather than a second vector compare of the sector mask (ONE_EXIT: the only exit;
+    // the lookup count then counts the steps that go on, and ray_trace adds the hit's)
+    if (ONE_EXIT) {
+        if (!inb || solid) return false;
+        r.iters++;
+        asm volatile("" : "+v"(r.iters));
+    } else if (solid) {
         *hit_slot = slot1 - 1;
         hit = true;
         return false;
@@ -283,15 +320,16 @@ __device__ __forceinline__ bool ray_trace(const WorldView& w, RayState& r, uint3
             if (!ray_advance<EXACT, CUBIC, ORIGIN>(w, r, v, hit, &unused)) break;
         return hit;
     }
-    // The fast loop carries no hit flag (its per-iteration lane-mask bookkeeping costs
-    // scalar instructions, and one scalar unit serves four SIMDs): the exit is told
-    // apart after the loop, as a hit leaves v on the solid voxel, a miss outside the
-    // world, and the cap on a voxel of the empty cell it was stepping over (every entry
-    // point rejects a cap of 0, so v is set by at least one step).
+    // The fast loop has one exit and no hit flag (see ONE_EXIT above): a hit leaves v on
+    // the solid voxel, a miss outside the world, and the cap on a voxel of the empty cell
+    // it was stepping over (every entry point rejects a cap of 0, so v is set by at least
+    // one step).
     for (uint32_t it = 0; it < max_iters; it++)
-        if (!ray_advance<EXACT, CUBIC, ORIGIN>(w, r, v, hit, &unused)) break;
+        if (!ray_advance<EXACT, CUBIC, ORIGIN, true>(w, r, v, hit, &unused)) break;
     uint32_t slot;
-    return solid_voxel_at(w, v, slot);
+    bool h = solid_voxel_at(w, v, slot);
+    r.iters += h ? 1u : 0u;
+    return h;
 }
 
 template <bool CUBIC>
