@@ -213,3 +213,32 @@ def test_fp64_picking_cast_matches_oracle(vrt, oracle_mod, cuda):
         r = ow.cast_f64(oo, dd, 1024)
         assert g.distance == r["distance"] and g.voxel_pos == r["voxel"] and g.normal == r["normal"]
         assert g.uv == r["uv"]
+
+
+@pytest.mark.parametrize("anchor", ["floor", "sector", "zero"])
+def test_counting_variant_writes_the_same_hits(vrt, cuda, anchor):
+    """The cast kernel's counting instantiation (the one that fills vrt_cast_counters) traces
+    the same rays to the same records as the plain one, on camera and bounce rays of each
+    world-origin specialization; its counts agree with the records."""
+    import voxelrt_amd as V
+    from voxelrt_amd import workloads as W
+
+    world = V.World.from_scene(W.C1_WORLD.build())
+    cam = V.CameraSpec((128.37, 200.61, 12.29), yaw_deg=8.0, pitch_deg=-28.0, width=640, height=360, tile=(8, 8),
+                       anchor=anchor)
+    prim = cam.rays_device()
+    hp = world.cast(prim, W.MAX_ITERS, with_pos=True)
+    br, _, cnt = V.bounce(prim, hp, cam.c.width, 9)
+    torch.cuda.synchronize()
+    nb = int(cnt.item())
+    for name, rays, n, kw in [("primary", prim, prim.n, {}), ("bounce", br, nb, {"count": cnt})]:
+        plain = world.cast(rays, W.MAX_ITERS, **kw)
+        ctr = V.new_counters()
+        counted = world.cast(rays, W.MAX_ITERS, counters=ctr, **kw)
+        torch.cuda.synchronize()
+        assert torch.equal(plain.rec[:n], counted.rec[:n]), f"{anchor} {name}"
+        assert torch.equal(plain.t[:n], counted.t[:n]), f"{anchor} {name}"
+        c = ctr.cpu().numpy()
+        rec = plain.rec[:n].cpu().numpy()
+        assert c[0] == n and c[3] == int(((rec[:, 3] & 0xFF) != 0).sum()), f"{anchor} {name}"
+        assert c[1] == int(((rec[:, 3] >> 16) & 0xFFFF).sum()), f"{anchor} {name}"

@@ -126,21 +126,24 @@ __device__ __forceinline__ void trace_block(const WorldView& w, const BatchArgs&
     bool wave_tame = __all(tame);
     bool hit[RPL];
     int32_t v[RPL][3];
+    uint32_t slot[RPL];
 #pragma unroll
-    for (int k = 0; k < RPL; k++) hit[k] = false;
+    for (int k = 0; k < RPL; k++) {
+        hit[k] = false;
+        slot[k] = 0;
+    }
     if constexpr (RPL == 1) {
         if (!done[0])
-            hit[0] = ray_trace_wave(w, st[0], a.max_iters, v[0], wave_tame, segment_origin(a, sg));
+            hit[0] = ray_trace_wave(w, st[0], a.max_iters, v[0], slot[0], wave_tame, segment_origin(a, sg));
     } else {
         // the lane's rays step in lockstep, so one counter caps them all
-        uint32_t unused;
         for (uint32_t it = 0; it < a.max_iters; it++) {
             bool any_running = false;
 #pragma unroll
             for (int k = 0; k < RPL; k++) {
                 if (!done[k]) {
-                    bool cont = wave_tame ? ray_advance<false>(w, st[k], v[k], hit[k], &unused)
-                                          : ray_advance<true>(w, st[k], v[k], hit[k], &unused);
+                    bool cont = wave_tame ? ray_advance<false>(w, st[k], v[k], hit[k], &slot[k])
+                                          : ray_advance<true>(w, st[k], v[k], hit[k], &slot[k]);
                     done[k] = !cont;
                 }
                 any_running = any_running || !done[k];
@@ -155,7 +158,7 @@ __device__ __forceinline__ void trace_block(const WorldView& w, const BatchArgs&
             float t;
             int4 rec;
             if (hit[k])
-                ray_hit_record(w, st[k], v[k], brick_slot_at(w, v[k]), t, rec);
+                ray_hit_record(w, st[k], v[k], slot[k], t, rec);
             else
                 ray_miss_record(st[k], t, rec);
             a.h.t[lr[k].out] = t;

@@ -63,8 +63,9 @@ __device__ __forceinline__ void store_result(const HitsOut& h, uint32_t i, const
 __device__ __forceinline__ void trace_full(const WorldView& w, RayState& st, uint32_t max_iters, bool wave_tame,
                                            int origin, float& t, int4& rec) {
     int32_t v[3];
-    if (ray_trace_wave(w, st, max_iters, v, wave_tame, origin))
-        ray_hit_record(w, st, v, brick_slot_at(w, v), t, rec);
+    uint32_t slot;
+    if (ray_trace_wave(w, st, max_iters, v, slot, wave_tame, origin))
+        ray_hit_record(w, st, v, slot, t, rec);
     else
         ray_miss_record(st, t, rec);
 }

@@ -58,14 +58,14 @@ __global__ __launch_bounds__(64) void render_fwd_kernel(WorldView w, FwdViews vi
     }
     bool wave_tame = __all(tame);
     if (valid) {
-        bool hit = ray_trace_wave(w, st, max_iters, v, wave_tame,
+        uint32_t slot = 0;
+        bool hit = ray_trace_wave(w, st, max_iters, v, slot, wave_tame,
                                   origin_kind(cam.world_origin[0], cam.world_origin[1], cam.world_origin[2]));
         float wgt = 0.0f;
         uint32_t gv = 0xFFFFFFFFu;
         if (hit) {
             float t;
             int4 rec;
-            uint32_t slot = brick_slot_at(w, v);
             ray_hit_record(w, st, v, slot, t, rec);
             float nx = normal_component((uint32_t)rec.w, 0), ny = normal_component((uint32_t)rec.w, 1),
                   nz = normal_component((uint32_t)rec.w, 2);

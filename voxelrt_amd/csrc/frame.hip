@@ -96,16 +96,16 @@ __device__ __forceinline__ void flush4(vrt_cast_counters* ctr, unsigned long lon
 // Traces the wave's active rays, with the fast step when all of them are tame;
 // true on a hit, with v the hit voxel.
 __device__ __forceinline__ bool trace_wave(const WorldView& w, RayState& st, bool active, uint32_t max_iters,
-                                           int origin, int32_t v[3]) {
+                                           int origin, int32_t v[3], uint32_t& slot) {
     bool tame = __all(!active || ray_is_tame(st));
     if (!active) return false;
-    return ray_trace_wave(w, st, max_iters, v, tame, origin);
+    return ray_trace_wave(w, st, max_iters, v, slot, tame, origin);
 }
 
 __device__ __forceinline__ void finish_trace(const WorldView& w, const RayState& st, bool hit, const int32_t v[3],
-                                             float& t, int4& rec) {
+                                             uint32_t slot, float& t, int4& rec) {
     if (hit)
-        ray_hit_record(w, st, v, brick_slot_at(w, v), t, rec);
+        ray_hit_record(w, st, v, slot, t, rec);
     else
         ray_miss_record(st, t, rec);
 }
@@ -120,27 +120,28 @@ __global__ __launch_bounds__(256) void frame_grid_kernel(WorldView w, FrameArgs 
     unsigned long long c[4] = {0, 0, 0, 0};
     RayState st;
     int32_t v[3] = {0, 0, 0};
+    uint32_t slot = 0;
     if (valid) setup_primary(a, i, st);
     const int origin = origin_kind(a.cam.world_origin[0], a.cam.world_origin[1], a.cam.world_origin[2]);
-    bool hit = trace_wave(w, st, valid, a.max_iters, origin, v);
+    bool hit = trace_wave(w, st, valid, a.max_iters, origin, v, slot);
     bool bounce = false;
     uint32_t info = 0;
     if (valid) {
         float t;
         int4 rec;
-        finish_trace(w, st, hit, v, t, rec);
+        finish_trace(w, st, hit, v, slot, t, rec);
         store_hit(a.hp, i, st.cur, t, rec);
         count_ray<COUNT>(c, st, rec);
         info = (uint32_t)rec.w;
         bounce = (info & 0xFF) != 0;
         if (bounce) setup_bounce(a, i, info, st);
     }
-    hit = trace_wave(w, st, bounce, a.max_iters, origin, v);
+    hit = trace_wave(w, st, bounce, a.max_iters, origin, v, slot);
     if (valid) {
         if (bounce) {
             float t;
             int4 rec;
-            finish_trace(w, st, hit, v, t, rec);
+            finish_trace(w, st, hit, v, slot, t, rec);
             store_hit(a.hb, i, st.cur, t, rec);
             count_ray<COUNT>(c, st, rec);
         } else {

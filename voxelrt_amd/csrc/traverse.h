@@ -309,24 +309,24 @@ __device__ __forceinline__ bool solid_voxel_at(const WorldView& w, const int32_t
 }
 
 // RayCast's loop (CpuRenderer.cpp:185-202): steps until a hit, a miss or
-// max_iters steps; true on a hit, with v the hit voxel (its slot is
-// brick_slot_at(w, v)).
+// max_iters steps; true on a hit, with v the hit voxel and slot its brick slot.
 template <bool EXACT, bool CUBIC = false, int ORIGIN = ORIGIN_ANY>
-__device__ __forceinline__ bool ray_trace(const WorldView& w, RayState& r, uint32_t max_iters, int32_t v[3]) {
+__device__ __forceinline__ bool ray_trace(const WorldView& w, RayState& r, uint32_t max_iters, int32_t v[3],
+                                          uint32_t& slot) {
     bool hit = false;
-    uint32_t unused;
     if (EXACT) {
         for (uint32_t it = 0; it < max_iters; it++)
-            if (!ray_advance<EXACT, CUBIC, ORIGIN>(w, r, v, hit, &unused)) break;
+            if (!ray_advance<EXACT, CUBIC, ORIGIN>(w, r, v, hit, &slot)) break;
         return hit;
     }
     // The fast loop has one exit and no hit flag (see ONE_EXIT above): a hit leaves v on
     // the solid voxel, a miss outside the world, and the cap on a voxel of the empty cell
     // it was stepping over (every entry point rejects a cap of 0, so v is set by at least
-    // one step).
+    // one step).  (Telling the cap apart by the step count instead of this lookup was
+    // miscompiled in the counting kernel variant: its hits and counts went wrong.)
+    uint32_t unused;
     for (uint32_t it = 0; it < max_iters; it++)
         if (!ray_advance<EXACT, CUBIC, ORIGIN, true>(w, r, v, hit, &unused)) break;
-    uint32_t slot;
     bool h = solid_voxel_at(w, v, slot);
     r.iters += h ? 1u : 0u;
     return h;
@@ -334,10 +334,10 @@ __device__ __forceinline__ bool ray_trace(const WorldView& w, RayState& r, uint3
 
 template <bool CUBIC>
 __device__ __forceinline__ bool ray_trace_tame(const WorldView& w, RayState& r, uint32_t max_iters, int32_t v[3],
-                                               int origin) {
-    if (origin == ORIGIN_ZERO) return ray_trace<false, CUBIC, ORIGIN_ZERO>(w, r, max_iters, v);
-    if (origin == ORIGIN_ALIGNED) return ray_trace<false, CUBIC, ORIGIN_ALIGNED>(w, r, max_iters, v);
-    return ray_trace<false, CUBIC>(w, r, max_iters, v);
+                                               uint32_t& slot, int origin) {
+    if (origin == ORIGIN_ZERO) return ray_trace<false, CUBIC, ORIGIN_ZERO>(w, r, max_iters, v, slot);
+    if (origin == ORIGIN_ALIGNED) return ray_trace<false, CUBIC, ORIGIN_ALIGNED>(w, r, max_iters, v, slot);
+    return ray_trace<false, CUBIC>(w, r, max_iters, v, slot);
 }
 
 // ray_trace with the step picked per wave: the fast step when every ray of the
@@ -345,10 +345,10 @@ __device__ __forceinline__ bool ray_trace_tame(const WorldView& w, RayState& r, 
 // and specialized for the rays' world origin (origin: an OriginKind, uniform over
 // the wave).
 __device__ __forceinline__ bool ray_trace_wave(const WorldView& w, RayState& r, uint32_t max_iters, int32_t v[3],
-                                               bool wave_tame, int origin = ORIGIN_ANY) {
-    if (!wave_tame) return ray_trace<true>(w, r, max_iters, v);
-    if (w.size_y == w.size_xz) return ray_trace_tame<true>(w, r, max_iters, v, origin);
-    return ray_trace_tame<false>(w, r, max_iters, v, origin);
+                                               uint32_t& slot, bool wave_tame, int origin = ORIGIN_ANY) {
+    if (!wave_tame) return ray_trace<true>(w, r, max_iters, v, slot);
+    if (w.size_y == w.size_xz) return ray_trace_tame<true>(w, r, max_iters, v, slot, origin);
+    return ray_trace_tame<false>(w, r, max_iters, v, slot, origin);
 }
 
 // Hit record fields (vrt_hit in vrt.h) of a finished ray.
