@@ -322,13 +322,26 @@ __device__ __forceinline__ bool ray_trace(const WorldView& w, RayState& r, uint3
     // The fast loop has one exit and no hit flag (see ONE_EXIT above): a hit leaves v on
     // the solid voxel, a miss outside the world, and the cap on a voxel of the empty cell
     // it was stepping over (every entry point rejects a cap of 0, so v is set by at least
-    // one step).  (Telling the cap apart by the step count instead of this lookup was
-    // miscompiled in the counting kernel variant: its hits and counts went wrong.)
+    // one step); r.iters counts the steps that went on, max_iters only after the cap.
     uint32_t unused;
     for (uint32_t it = 0; it < max_iters; it++)
         if (!ray_advance<EXACT, CUBIC, ORIGIN, true>(w, r, v, hit, &unused)) break;
-    bool h = solid_voxel_at(w, v, slot);
-    r.iters += h ? 1u : 0u;
+    if (ORIGIN == ORIGIN_ANY) {
+        // here the step overwrites v with the cell corner, and a test of the step count
+        // and v costs the loop 7 VALU instructions in exit copies: look the voxel up
+        bool h = solid_voxel_at(w, v, slot);
+        r.iters += h ? 1u : 0u;
+        return h;
+    }
+    // a hit is a ray stopped inside the world before the cap; the count and v pass through
+    // an empty asm statement, without which the counting kernel's instance of this test
+    // was miscompiled (its hits and counts went wrong; the plain instance was right)
+    uint32_t steps = r.iters;
+    asm volatile("" : "+v"(steps), "+v"(v[0]), "+v"(v[1]), "+v"(v[2]));
+    bool h = steps < max_iters && (CUBIC ? (uint32_t)(v[0] | v[1] | v[2]) < w.size_xz
+                                         : ((uint32_t)(v[0] | v[2]) < w.size_xz && (uint32_t)v[1] < w.size_y));
+    if (h) slot = brick_slot_at(w, v);
+    r.iters = steps + (h ? 1u : 0u);
     return h;
 }
 
