@@ -179,7 +179,7 @@ __global__ __launch_bounds__(1024) void bounce_write_rows_kernel(const int4* __r
     // hits of the blocks before this one (block 0: of all blocks, the total)
     const uint32_t lim = b == 0 ? nblocks : b;
     uint32_t before = 0;
-    for (uint32_t k = t; k < lim; k += 1024) before += block_counts[k];
+    for (uint32_t k = t; k < lim; k += 1024) before += block_counts[k];  // kGenMaxBlocks / 1024 loads at most
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) before += __shfl_xor(before, off, 64);
     if (lane == 0) wave_sums[wave] = before;
@@ -195,7 +195,7 @@ __global__ __launch_bounds__(1024) void bounce_write_rows_kernel(const int4* __r
     __syncthreads();
     if (t < 64) {
         // exclusive scan of the 64 (row, wave) counts in row-major order = primary order
-        uint32_t c = row_base[t], incl = c;
+        uint32_t c = t < kGenRows * 16 ? row_base[t] : 0u, incl = c;
 #pragma unroll
         for (uint32_t off = 1; off < 64; off <<= 1) {
             uint32_t y = __shfl_up(incl, off, 64);
@@ -208,7 +208,7 @@ __global__ __launch_bounds__(1024) void bounce_write_rows_kernel(const int4* __r
             if (t == 0) *count = base;  // block 0 summed every block: the total
             base = 0;
         }
-        row_base[t] = base + incl - c;
+        if (t < kGenRows * 16) row_base[t] = base + incl - c;
     }
     __syncthreads();
 #pragma unroll
