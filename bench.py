@@ -220,6 +220,8 @@ def main() -> int:
     ap.add_argument("--anchor", default="sector", choices=["floor", "sector"],
                     help="integer world origin of the camera rays: the camera's sector corner (default; the "
                          "aligned-origin step) or floor(eye)")
+    ap.add_argument("--event-every", type=int, default=8,
+                    help="HIP events around the launches of every n-th timed step (the roofline's launch times)")
     ap.add_argument("--cpu-runs", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
@@ -334,26 +336,35 @@ def main() -> int:
     cp_h, cb_h = cp.cpu().numpy(), cb.cpu().numpy()
     assert int(cp_h[0]) == n and int(cb_h[0]) == nb, (cp_h, cb_h, n, nb)
 
-    # timed region: K steps, HIP events around every cast launch on the launch stream
+    # timed region: K steps; every args.event_every-th step (and the first) has HIP events
+    # around each of its launches on the launch stream (an event between two launches costs
+    # the stream ~3.5 us, so events on every step would add 5% to the step)
     K = args.steps
-    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(K)]
+    every = max(1, args.event_every)
+    sampled = [k for k in range(K) if k % every == 0]
+    ev = {k: [torch.cuda.Event(enable_timing=True) for _ in range(4)] for k in sampled}
     torch.cuda.synchronize()
     t_start = time.perf_counter()
     for k in range(K):
-        ev[k][0].record(stream)
+        e = ev.get(k)
+        if e is None:
+            cast_primary()
+            gen_bounce()
+            cast_bounce()
+            continue
+        e[0].record(stream)
         cast_primary()
-        ev[k][1].record(stream)
+        e[1].record(stream)
         gen_bounce()
-        ev[k][2].record(stream)
+        e[2].record(stream)
         cast_bounce()
-        ev[k][3].record(stream)
+        e[3].record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t_start
-    step_ms = [ev[k][0].elapsed_time(ev[k][3]) for k in range(K)]
-    prim_ms = [ev[k][0].elapsed_time(ev[k][1]) for k in range(K)]
-    bounce_ms = [ev[k][2].elapsed_time(ev[k][3]) for k in range(K)]
-    gen_ms = [ev[k][1].elapsed_time(ev[k][2]) for k in range(K)]
-    gpu_s = sum(step_ms) / 1e3
+    step_ms = [ev[k][0].elapsed_time(ev[k][3]) for k in sampled]
+    prim_ms = [ev[k][0].elapsed_time(ev[k][1]) for k in sampled]
+    bounce_ms = [ev[k][2].elapsed_time(ev[k][3]) for k in sampled]
+    gen_ms = [ev[k][1].elapsed_time(ev[k][2]) for k in sampled]
     rays_per_step = n + nb
     value = rays_per_step * K / wall / 1e6
     ms_per_step = wall * 1e3 / K
@@ -369,6 +380,7 @@ def main() -> int:
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": None,
+        "launch_samples": len(sampled),
     }
     # achievable HBM bandwidth on this device, measured in the same run: a 1 GiB device-to-device
     # copy (reads and writes 1 GiB each)
@@ -394,7 +406,7 @@ def main() -> int:
         f"iters/ray {cp_h[1] / n:.2f}, descents/ray {cp_h[2] / n:.2f}, hit {cp_h[3] / n:.3f}")
     log(f"bounce  {nb} rays: {statistics.mean(bounce_ms):.3f} ms ({nb / statistics.mean(bounce_ms) / 1e3:.0f} Mrays/s), "
         f"iters/ray {cb_h[1] / max(nb, 1):.2f}, descents/ray {cb_h[2] / max(nb, 1):.2f}, hit {cb_h[3] / max(nb, 1):.3f}")
-    log(f"bounce generation {statistics.mean(gen_ms):.3f} ms; step {statistics.mean(step_ms):.3f} ms (events), "
+    log(f"bounce generation {statistics.mean(gen_ms):.3f} ms; step {statistics.mean(step_ms):.3f} ms (events, {len(sampled)} sampled steps), "
         f"{ms_per_step:.3f} ms (wall); cast roofline {achieved:.0f} GB/s = {achieved / HBM_PEAK_GBS:.1%} of 8 TB/s, "
         f"{achieved / achievable:.1%} of the {achievable:.0f} GB/s a 1 GiB device copy reaches")
 
