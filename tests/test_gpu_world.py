@@ -203,3 +203,46 @@ def test_c3_spheres_parity(vrt, oracle_mod, cuda, sparse):
         assert_hits_equal(h.t.cpu().numpy(), h.rec.cpu().numpy(), ref["t"], ref["rec"], f"C3 {name}")
         hit = float(((ref["rec"][:, 3] & 0xFF) != 0).mean())
         print(f"C3 sparse={sparse} {name}: {rb.n} rays, {rate:.0f} Mrays/s, hit fraction {hit:.3f}", flush=True)
+
+
+def test_maximum_extent_world(vrt, cuda):
+    """The largest world (16384^3 voxels: 512^3 sectors, a 2 GiB sector table): one voxel near
+    the far corner is found along each axis from across the world, at each world-origin
+    specialization, at the closed-form distance to its entry face; a ray crossing the world
+    away from it misses.  The sync of one sector updates that record only (scatter path)."""
+    import voxelrt_amd as V
+
+    n = 1 << 14
+    world = V.World(14, 14)
+    assert world.info().num_sectors == 1 << 27
+    X, Y, Z = n - 4, n - 3, n - 2
+    si = (X >> 5) | (Z >> 5) << 9 | (Y >> 5) << 18
+    bi = ((X >> 3) & 3) | ((Z >> 3) & 3) << 2 | ((Y >> 3) & 3) << 4
+    brick = np.zeros((1, 512), np.uint8)
+    brick[0, (X & 7) | (Z & 7) << 3 | (Y & 7) << 6] = 9
+    world.sync(np.array([si]), np.array([1 << bi], np.uint64), brick)
+    e = 2.0 ** -20
+    cases = [  # absolute origin, direction, closed-form t (None: a miss), normal
+        ((X + 0.5, Y + 0.5, 0.5), (e, e, 1.0), Z - 0.5, (0, 0, -1)),
+        ((X + 0.5, 0.5, Z + 0.5), (e, 1.0, e), Y - 0.5, (0, -1, 0)),
+        ((0.5, Y + 0.5, Z + 0.5), (1.0, e, e), X - 0.5, (-1, 0, 0)),
+        ((n - 0.25, Y + 0.5, Z + 0.5), (-1.0, e, e), n - 0.25 - (X + 1), (1, 0, 0)),
+        ((0.5, 0.5, 0.5), (e, 1.0, e), None, None),
+    ]
+    absolute = np.array([c[0] for c in cases], np.float64).T
+    d = np.array([c[1] for c in cases], np.float32).T
+    # world origins of each specialization of the step: zero, a sector corner, any
+    for W in [(0, 0, 0), (X & ~31, Y & ~31, 0), (X - 3, 7, 5)]:
+        o = (absolute - np.array(W, np.float64)[:, None]).astype(np.float32)
+        assert np.array_equal(o.astype(np.float64) + np.array(W, np.float64)[:, None], absolute)
+        h = world.cast(V.RayBatch.from_numpy(o, d, W), 4096)
+        torch.cuda.synchronize()
+        t, rec = h.t.cpu().numpy(), h.rec.cpu().numpy()
+        for k, (_, _, tk, nk) in enumerate(cases):
+            info = int(rec[k, 3]) & 0xFFFFFFFF
+            if tk is None:
+                assert info & 0xFF == 0 and t[k] == -1.0, (W, k)
+                continue
+            normal = tuple(((info >> (8 + 2 * a)) & 3 ^ 2) - 2 for a in range(3))
+            assert (tuple(rec[k, :3]), info & 0xFF, normal) == ((X, Y, Z), 9, nk), (W, k)
+            assert t[k] == np.float32(tk), (W, k, t[k], tk)

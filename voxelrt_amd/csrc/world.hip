@@ -144,6 +144,13 @@ hipError_t launch_build_occupancy(vrt_world* w, const uint32_t* d_slots, uint32_
     return hipGetLastError();
 }
 
+// The records of the synced sectors (indices unique) into the sector table.
+__global__ __launch_bounds__(256) void scatter_records_kernel(uint4* __restrict__ sec, const uint4* __restrict__ rec,
+                                                              const uint32_t* __restrict__ idx, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) sec[idx[i]] = rec[i];
+}
+
 }  // namespace vrt
 
 using namespace vrt;
@@ -401,30 +408,54 @@ int vrt_world_sync(vrt_world* w, uint32_t nsec, const uint32_t* sector_idx, cons
         }
         for (const Run& r : runs)
             for (uint32_t k = 0; k < r.count; k++) slots.push_back(r.dst + k);
-        // device: bricks, occupancy, headers (in stream order)
-        for (const Run& r : runs)
-            VRT_HIP(hipMemcpyAsync(w->d_vox + (size_t)r.dst * 512, bricks + (size_t)r.src * 512, (size_t)r.count * 512,
-                                   hipMemcpyHostToDevice, s));
-        uint32_t* d_slots = nullptr;
-        if (!slots.empty()) {
-            VRT_HIP(hipMalloc(&d_slots, slots.size() * 4));
-            hipError_t e = hipMemcpyAsync(d_slots, slots.data(), slots.size() * 4, hipMemcpyHostToDevice, s);
-            if (e == hipSuccess) e = launch_build_occupancy(w, d_slots, (uint32_t)slots.size(), s);
-            if (e != hipSuccess) {
-                (void)hipStreamSynchronize(s);
-                (void)hipFree(d_slots);
-                return hip_fail(e, "occupancy build");
+        // device: bricks, occupancy, headers (in stream order).  One scratch allocation holds
+        // the slots whose occupancy is rebuilt and, when the synced sectors are few against the
+        // grid, their records for a scatter launch (the largest grid's table is 2 GiB).
+        const bool scatter = (uint64_t)nsec * 64 <= w->nsec;
+        const size_t slots_bytes = (slots.size() * 4 + 15) & ~(size_t)15;
+        const size_t tmp_bytes = slots_bytes + (scatter ? (size_t)nsec * 20 : 0);
+        std::vector<uint8_t> h_tmp(tmp_bytes);
+        std::memcpy(h_tmp.data(), slots.data(), slots.size() * 4);
+        if (scatter) {
+            uint4* rec = (uint4*)(h_tmp.data() + slots_bytes);
+            uint32_t* idx = (uint32_t*)(h_tmp.data() + slots_bytes + (size_t)nsec * 16);
+            for (uint32_t i = 0; i < nsec; i++) {
+                idx[i] = sector_idx[i];
+                rec[i] = sector_record(w->h_mask[sector_idx[i]], w->h_base[sector_idx[i]]);
             }
         }
-        std::vector<uint4> recs(w->nsec);
-        for (uint32_t i = 0; i < w->nsec; i++)
-            recs[i] = sector_record(w->h_mask[i], w->h_base[i]);
-        VRT_HIP(hipMemcpyAsync(w->d_sec, recs.data(), (size_t)w->nsec * 16, hipMemcpyHostToDevice, s));
-        VRT_HIP(hipMemcpyAsync(w->d_groups, w->h_groups.data(), w->h_groups.size() * 8, hipMemcpyHostToDevice, s));
-        hipError_t e = hipStreamSynchronize(s);
-        (void)hipFree(d_slots);
-        if (e != hipSuccess) return hip_fail(e, "vrt_world_sync: hipStreamSynchronize");
-        return VRT_OK;
+        uint8_t* d_tmp = nullptr;
+        if (tmp_bytes) VRT_HIP(hipMalloc(&d_tmp, tmp_bytes));
+        auto done = [&](hipError_t e, const char* what) -> int {
+            hipError_t e2 = hipStreamSynchronize(s);  // the scratch is in use until the stream gets here
+            (void)hipFree(d_tmp);
+            if (e == hipSuccess) e = e2;
+            return e == hipSuccess ? VRT_OK : hip_fail(e, what);
+        };
+        std::vector<uint4> recs;  // the whole table, when it is uploaded: alive until done()
+        hipError_t e = hipSuccess;
+        for (const Run& r : runs)
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(w->d_vox + (size_t)r.dst * 512, bricks + (size_t)r.src * 512, (size_t)r.count * 512,
+                                   hipMemcpyHostToDevice, s);
+        if (e == hipSuccess && tmp_bytes) e = hipMemcpyAsync(d_tmp, h_tmp.data(), tmp_bytes, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess && !slots.empty())
+            e = launch_build_occupancy(w, (const uint32_t*)d_tmp, (uint32_t)slots.size(), s);
+        if (e != hipSuccess) return done(e, "vrt_world_sync: bricks and occupancy");
+        if (scatter) {
+            scatter_records_kernel<<<(nsec + 255) / 256, 256, 0, s>>>(
+                w->d_sec, (const uint4*)(d_tmp + slots_bytes), (const uint32_t*)(d_tmp + slots_bytes + (size_t)nsec * 16),
+                nsec);
+            e = hipGetLastError();
+        } else {
+            recs.resize(w->nsec);
+            for (uint32_t i = 0; i < w->nsec; i++)
+                recs[i] = sector_record(w->h_mask[i], w->h_base[i]);
+            e = hipMemcpyAsync(w->d_sec, recs.data(), (size_t)w->nsec * 16, hipMemcpyHostToDevice, s);
+        }
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(w->d_groups, w->h_groups.data(), w->h_groups.size() * 8, hipMemcpyHostToDevice, s);
+        return done(e, "vrt_world_sync: sector records");
     });
 }
 
