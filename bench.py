@@ -222,6 +222,9 @@ def main() -> int:
                          "aligned-origin step) or floor(eye)")
     ap.add_argument("--event-every", type=int, default=8,
                     help="HIP events around the launches of every n-th timed step (the roofline's launch times)")
+    ap.add_argument("--only", default="all", choices=["all", "batched", "diff"],
+                    help="run one section alone and print its result (profiling runs); the default runs the "
+                         "metric with every section")
     ap.add_argument("--cpu-runs", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
@@ -240,6 +243,18 @@ def main() -> int:
 
     import voxelrt_amd as V
     from voxelrt_amd import workloads as W
+
+    if args.only != "all":
+        dev = torch.device("cuda:0")
+        torch.cuda.set_device(dev)
+        if args.only == "diff":
+            res = bench_differentiable(dev)
+        else:
+            log("building the 1024^3 terrain + caves world")
+            world = V.World.from_scene(W.C2_WORLD.build())
+            res = bench_batched(world, W.MAX_ITERS, dev)
+        print(json.dumps({"section": args.only, **res}), flush=True)
+        return 0
 
     tile = tuple(int(x) for x in args.tile.split("x"))
     sched = 1 if args.schedule == "grid" else 2
