@@ -33,8 +33,9 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
-# algorithmic bytes: ray in, hit out, per-lookup sector mask,
-# per-descent base slot + occupancy mask, per-hit voxel byte
+# algorithmic bytes per ray: ray in (6 x f32), hit out (t + 16-byte record; + 12 B of
+# position for primaries), 8 B sector mask per lookup, 4 B base slot + 8 B occupancy mask per
+# descent into a brick, 1 B voxel per hit
 B_RAY_IN = 24
 B_HIT_OUT = 20
 B_POS_OUT = 12
@@ -234,8 +235,8 @@ def main() -> int:
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (rays shard across GPUs "
-              "with a replicated world, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported here (a multi-GPU cast "
+              "would replicate the world on every GPU and split the rays between them)", file=sys.stderr)
         return 2
 
     import numpy as np
