@@ -11,6 +11,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "traverse.h"
 #include "vrt_internal.h"
 #include "vrt_math.h"
 
@@ -21,9 +22,9 @@ __device__ __forceinline__ bool fine_bit(uint64_t m, uint32_t idx) { return (m >
 // getStepPos (:92-131).  Returns true when v is solid (or, for a coarse step, when v
 // was moved to the first solid voxel of its non-empty 4x4x4 cell); otherwise moves v
 // to the far corner, in ray direction, of the largest empty aligned cell.
-template <bool LUT>
-__device__ __forceinline__ bool shader_step(const ShaderView& sv, int32_t v[3], const float d[3], uint32_t oct,
-                                            bool coarse) {
+template <bool LUT, bool TAME>
+__device__ __forceinline__ bool shader_step(const ShaderView& sv, int32_t v[3], const float d[3],
+                                            const int32_t dm[3], uint32_t oct, bool coarse) {
     const WorldView& w = sv.w;
     uint32_t ux = (uint32_t)v[0], uy = (uint32_t)v[1], uz = (uint32_t)v[2];
     uint32_t sx = ux >> 5, sy = uy >> 5, sz = uz >> 5;
@@ -66,9 +67,13 @@ __device__ __forceinline__ bool shader_step(const ShaderView& sv, int32_t v[3], 
         v[2] = (v[2] & ~3) | (int32_t)((bit >> 2) & 3u);
         return true;
     }
-    int32_t cm = lod - 1;
+    // d < 0: v & ~cm, else v | cm (the fast loop: one bitfield insert of dm, all ones unless
+    // d < 0)
+    uint32_t cm = (uint32_t)lod - 1u;
 #pragma unroll
-    for (int a = 0; a < 3; a++) v[a] = d[a] < 0.0f ? (v[a] & ~cm) : (v[a] | cm);
+    for (int a = 0; a < 3; a++)
+        v[a] = TAME ? (int32_t)bitfield_insert(~cm, (uint32_t)v[a], (uint32_t)dm[a])
+                    : (d[a] < 0.0f ? (int32_t)((uint32_t)v[a] & ~cm) : (int32_t)((uint32_t)v[a] | cm));
     return false;
 }
 
@@ -103,25 +108,17 @@ struct ShaderHit {
     uint32_t mat;   // material id, 0 on a miss
 };
 
-// rayCast (COARSE = false, :162-204) or rayCastCoarse (COARSE = true, :206-243).
-template <bool COARSE, bool LUT>
-__device__ __forceinline__ bool shader_trace(const ShaderView& sv, const float o[3], const float d[3],
-                                             const int32_t W[3], uint32_t cap, ShaderHit& out) {
+// The loop of rayCast / rayCastCoarse.  TAME (every active ray of the wave has finite
+// 1/d and tStart, |origin| < 2^20 and 0.25 <= |d|^2 <= 4): positions stay far inside int32
+// range for any cap (a step moves less than 128 sqrt(3) / 0.5 voxels) and tmin is never NaN,
+// so floor is one V_CVT_FLR_I32_F32 instead of floor2i's range test.
+template <bool COARSE, bool LUT, bool TAME>
+__device__ __forceinline__ bool shader_loop(const ShaderView& sv, const float d[3], const float org[3],
+                                            const float inv[3], const float ts[3], const int32_t W[3],
+                                            const int32_t dm[3], uint32_t oct, uint32_t cap, int32_t v[3],
+                                            float side[3], float cur[3], float& tmin, uint32_t& it) {
     const WorldView& w = sv.w;
-    float start[3], org[3], inv[3], ts[3], side[3] = {0.0f, 0.0f, 0.0f}, cur[3] = {0.0f, 0.0f, 0.0f};
-    clip_ray(w, o, d, W, start);
-    int32_t v[3];
-#pragma unroll
-    for (int a = 0; a < 3; a++) {
-        org[a] = COARSE ? start[a] : o[a];
-        inv[a] = 1.0f / d[a];
-        ts[a] = ((d[a] < 0.0f ? 0.0f : 1.0f) - org[a]) * inv[a];
-        v[a] = (int32_t)((uint32_t)W[a] + (uint32_t)floor2i(start[a]));
-    }
-    const uint32_t oct = (d[0] < 0.0f ? 0u : 1u) + (d[1] < 0.0f ? 0u : 2u) + (d[2] < 0.0f ? 0u : 4u);
-    float tmin = 0.0f;
     bool hit = false;
-    uint32_t it = 0;
     for (; it < cap; it++) {
 #pragma unroll
         for (int a = 0; a < 3; a++)
@@ -134,43 +131,80 @@ __device__ __forceinline__ bool shader_trace(const ShaderView& sv, const float o
 #pragma unroll
         for (int a = 0; a < 3; a++) {
             cur[a] = fmaf(tmin, d[a], org[a]);
-            v[a] = (int32_t)((uint32_t)W[a] + (uint32_t)floor2i(cur[a]));
+            v[a] = (int32_t)((uint32_t)W[a] + (uint32_t)(TAME ? floor_i32_in_range(cur[a]) : floor2i(cur[a])));
         }
         if (!((uint32_t)(v[0] | v[2]) < w.size_xz && (uint32_t)v[1] < w.size_y)) break;
-        if (shader_step<LUT>(sv, v, d, oct, COARSE && it > 30)) {
+        if (shader_step<LUT, TAME>(sv, v, d, dm, oct, COARSE && it > 30)) {
             hit = true;
             break;
         }
     }
-    out.iters = it;
-    if (!hit) {
-        out.t = -1.0f;
-        out.cur[0] = out.cur[1] = out.cur[2] = 0.0f;
-        out.v[0] = out.v[1] = out.v[2] = 0;
-        out.code = 0;
-        out.mat = 0;
-        return false;
-    }
-    uint32_t code = 0;
-#pragma unroll
-    for (int a = 0; a < 3; a++)
-        if (tmin >= side[a]) code |= (d[a] < 0.0f ? 1u : (d[a] > 0.0f ? 3u : 0u)) << (2 * a);
-    uint32_t ux = (uint32_t)v[0], uy = (uint32_t)v[1], uz = (uint32_t)v[2];
-    uint32_t si = (ux >> 5) | (uz >> 5) << w.shift_xz | (uy >> 5) << (2 * w.shift_xz);
-    uint4 sec = w.sec[si];
-    uint64_t m = (uint64_t)sec.x | (uint64_t)sec.y << 32;
-    uint32_t bi = __builtin_amdgcn_ubfe(ux, 3, 2) | __builtin_amdgcn_ubfe(uz, 3, 2) << 2 |
-                  __builtin_amdgcn_ubfe(uy, 3, 2) << 4;
-    uint32_t slot = sec.z + (uint32_t)__popcll(m & ((1ull << bi) - 1ull));
-    out.mat = w.vox[(size_t)slot * 512 + ((ux & 7u) | (uz & 7u) << 3 | (uy & 7u) << 6)];
-    out.t = tmin;
+    return hit;
+}
+
+// rayCast (COARSE = false, :162-204) or rayCastCoarse (COARSE = true, :206-243).
+template <bool COARSE, bool LUT>
+__device__ __forceinline__ bool shader_trace(const ShaderView& sv, const float o[3], const float d[3],
+                                             const int32_t W[3], uint32_t cap, ShaderHit& out) {
+    const WorldView& w = sv.w;
+    float start[3], org[3], inv[3], ts[3], side[3] = {0.0f, 0.0f, 0.0f}, cur[3] = {0.0f, 0.0f, 0.0f};
+    clip_ray(w, o, d, W, start);
+    int32_t v[3], dm[3];
+    bool tame = true;
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-        out.cur[a] = cur[a];
-        out.v[a] = v[a];
+        org[a] = COARSE ? start[a] : o[a];
+        inv[a] = 1.0f / d[a];
+        ts[a] = ((d[a] < 0.0f ? 0.0f : 1.0f) - org[a]) * inv[a];
+        v[a] = (int32_t)((uint32_t)W[a] + (uint32_t)floor2i(start[a]));
+        dm[a] = d[a] < 0.0f ? 0 : -1;
+        tame = tame && fabsf(inv[a]) < INFINITY && fabsf(ts[a]) < INFINITY && fabsf(org[a]) < 1048576.0f;
     }
+    float len2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    tame = tame && len2 >= 0.25f && len2 <= 4.0f;
+    const uint32_t oct = (d[0] < 0.0f ? 0u : 1u) + (d[1] < 0.0f ? 0u : 2u) + (d[2] < 0.0f ? 0u : 4u);
+    float tmin = 0.0f;
+    uint32_t it = 0;
+    // the fast loop for rayCast; rayCastCoarse's short traces (cap 96, mostly shadow and bounce
+    // rays inside the renderer) measured no gain from it, and its second loop copy slowed
+    // the renderer's kernel, which holds three traces
+    bool hit = !COARSE && __all(tame)
+                   ? shader_loop<COARSE, LUT, true>(sv, d, org, inv, ts, W, dm, oct, cap, v, side, cur, tmin, it)
+                   : shader_loop<COARSE, LUT, false>(sv, d, org, inv, ts, W, dm, oct, cap, v, side, cur, tmin, it);
+    // the record is built in locals and stored once, so the result stays in registers
+    uint32_t code = 0, mat = 0;
+    float t = -1.0f;
+    int32_t hv[3] = {0, 0, 0};
+    float hc[3] = {0.0f, 0.0f, 0.0f};
+    if (hit) {
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+            if (tmin >= side[a]) code |= (d[a] < 0.0f ? 1u : (d[a] > 0.0f ? 3u : 0u)) << (2 * a);
+        uint32_t ux = (uint32_t)v[0], uy = (uint32_t)v[1], uz = (uint32_t)v[2];
+        uint32_t si = (ux >> 5) | (uz >> 5) << w.shift_xz | (uy >> 5) << (2 * w.shift_xz);
+        uint4 sec = w.sec[si];
+        uint64_t m = (uint64_t)sec.x | (uint64_t)sec.y << 32;
+        uint32_t bi = __builtin_amdgcn_ubfe(ux, 3, 2) | __builtin_amdgcn_ubfe(uz, 3, 2) << 2 |
+                      __builtin_amdgcn_ubfe(uy, 3, 2) << 4;
+        uint32_t slot = sec.z + (uint32_t)__popcll(m & ((1ull << bi) - 1ull));
+        mat = w.vox[(size_t)slot * 512 + ((ux & 7u) | (uz & 7u) << 3 | (uy & 7u) << 6)];
+        t = tmin;
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            hc[a] = cur[a];
+            hv[a] = v[a];
+        }
+    }
+    out.iters = it;
+    out.t = t;
     out.code = code;
-    return true;
+    out.mat = mat;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        out.cur[a] = hc[a];
+        out.v[a] = hv[a];
+    }
+    return hit;
 }
 
 }  // namespace vrt
