@@ -14,7 +14,9 @@
  * Every entry point returns a vrt_status; on failure vrt_last_error() gives a
  * thread-local message.  No C++ exception crosses this boundary.  Device work
  * is asynchronous on the given stream (a hipStream_t, NULL = the null stream);
- * buffers passed to casts are device pointers owned by the caller.
+ * buffers passed to casts are device pointers owned by the caller.  A call with
+ * nothing to do (no rays, views or pixels) checks its options, launches nothing
+ * and returns VRT_OK; its buffer pointers may then be NULL.
  *
  * Reference interfaces replaced (paths relative to the VoxelRT source tree):
  *   vrt_world_*            GpuVoxelStorage / FlatVoxelStorage SyncBuffers
