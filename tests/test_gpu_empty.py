@@ -57,6 +57,13 @@ def test_zero_rays_bounce_generation_writes_zero_count(vrt, cuda, small_world):
                                         count.data_ptr(), scratch.data_ptr(), None))
     torch.cuda.synchronize()
     assert int(count.item()) == 0
+    # with nothing to generate, only the count pointer is needed
+    count.fill_(55)
+    nul = L.RayBuffers(0, 0, 0, 0, 0, 0)
+    L.check(L.lib().vrt_bounce_generate(C.byref(r), C.byref(L.Hits(0, 0, 0)), 64, 1, C.byref(nul), None,
+                                        count.data_ptr(), None, None))
+    torch.cuda.synchronize()
+    assert int(count.item()) == 0
     # and a cast given that device count of 0 traces nothing
     prim = small_world[1].rays_device()
     hb = V.HitBatch.empty(prim.n)

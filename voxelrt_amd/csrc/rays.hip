@@ -262,14 +262,15 @@ size_t vrt_bounce_scratch_bytes(uint32_t n) {
 int vrt_bounce_generate(const vrt_rays* primary, const vrt_hits* hits, uint32_t width, uint32_t seed,
                         vrt_ray_buffers* out, uint32_t* src, uint32_t* count, void* scratch, void* stream) {
     return guarded([&]() -> int {
-        if (!primary || !hits || !out || !src || !count || !scratch)
+        if (!primary || !hits || !out || !count)
             return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_bounce_generate: null argument");
-        if (!hits->pos) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_bounce_generate: hits need positions");
         if (width == 0) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_bounce_generate: width must be > 0");
-        if (!out->ox || !out->oy || !out->oz || !out->dx || !out->dy || !out->dz)
-            return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_bounce_generate: null ray buffer");
         hipStream_t s = (hipStream_t)stream;
         uint32_t n = primary->n;
+        if (n && (!src || !scratch || !hits->rec)) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_bounce_generate: null argument");
+        if (n && !hits->pos) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_bounce_generate: hits need positions");
+        if (n && (!out->ox || !out->oy || !out->oz || !out->dx || !out->dy || !out->dz))
+            return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_bounce_generate: null ray buffer");
         uint32_t nb = (n + kBounceBlock - 1) / kBounceBlock;
         uint32_t* blocks = (uint32_t*)scratch;
         if (n == 0) {
