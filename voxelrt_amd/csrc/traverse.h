@@ -33,11 +33,12 @@
 //
 // ONE_EXIT (the fast loop): the lookup is made before the bounds test, so a step
 // leaves the loop at one place for both ends of a ray.  A lane that has left the
-// world reads a sector index past the records, which the structured buffer load
-// answers with 0 (an empty sector), and it does not descend.  The loop keeps no hit
-// flag: ray_trace tells the exits apart with one lookup after the loop.  Each exit
-// and flag costs lane-mask bookkeeping on the scalar unit, which one CU's four SIMDs
-// share, so the one-exit step issues 25 scalar instructions against 36.
+// world reads whatever record its index names (an index past the records reads 0:
+// the structured buffer load checks the range) and does not descend, so it never
+// reads an occupancy mask.  The loop keeps no hit flag: ray_trace tells the exits
+// apart after the loop (by the step count, or by one lookup at any origin).  Each
+// exit and flag costs lane-mask bookkeeping on the scalar unit, which one CU's four
+// SIMDs share, so the one-exit step issues 25 scalar instructions against 36.
 #pragma once
 
 #include <hip/hip_runtime.h>
