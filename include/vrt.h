@@ -267,6 +267,7 @@ typedef struct vrt_shader_opts {
     uint32_t reserved;      /* 0 */
 } vrt_shader_opts;
 
+/* At most 2^32 - 64 rays per call. */
 int vrt_cast_shader(const vrt_world* world, const vrt_rays* rays, const vrt_shader_opts* opts,
                     vrt_hits* hits, void* stream);
 
@@ -312,7 +313,8 @@ typedef struct vrt_camera {
     uint32_t tile_w, tile_h;   /* 1x1 = plain row-major */
 } vrt_camera;
 
-/* Generate the primary rays of a camera into device buffers (n = width*height). */
+/* Generate the primary rays of a camera into device buffers (n = width*height, at most
+ * 2^32 - 256). */
 int vrt_camera_rays(const vrt_camera* cam, vrt_ray_buffers* out, void* stream);
 
 /* ---------------------------------------------------------------- fused frame */
@@ -452,7 +454,8 @@ int vrt_batch_autotune(const vrt_world* world, const vrt_batch* batch, const cha
  * this: the array's length is not passed).
  * image: [sum of width*height over views] * 3, views concatenated, pixels in
  * each camera's ray order.  saved_voxel (u32, ~0 on a miss) and saved_weight
- * (f32) are per pixel and are consumed by the backward pass.
+ * (f32) are per pixel and are consumed by the backward pass.  At most 2^32 - 4096
+ * pixels over all views.
  */
 int vrt_render_fwd(const vrt_world* world, const vrt_camera* cameras, uint32_t nviews,
                    uint32_t max_iters, const float light_dir[3], float intensity,
@@ -460,7 +463,8 @@ int vrt_render_fwd(const vrt_world* world, const vrt_camera* cameras, uint32_t n
                    float* saved_weight, void* stream);
 
 /* grad_albedo[v][c] += sum over pixels hitting v of grad_image[p][c] * saved_weight[p].
- * grad_albedo has the albedo's length; the world is unchanged since the forward pass. */
+ * grad_albedo has the albedo's length; the world is unchanged since the forward pass.
+ * npixels is at most 2^32 - 256. */
 int vrt_render_bwd(const vrt_world* world, uint32_t npixels, const float* grad_image,
                    const uint32_t* saved_voxel, const float* saved_weight,
                    float* grad_albedo, void* stream);

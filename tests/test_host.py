@@ -114,3 +114,36 @@ def test_world_create_without_device_reports_no_device(vrt):
     with pytest.raises(vrt.VrtError) as e:
         vrt.World(8, 8)
     assert e.value.status == 4
+
+
+def test_oversized_launches_are_rejected_before_any_device_work(vrt):
+    """Counts whose 32-bit grid (blocks x lanes) would wrap are refused by the
+    argument checks, which run before the world is read or a kernel launched."""
+    import torch
+
+    from voxelrt_amd import _lib as L
+
+    if torch.cuda.is_available():
+        pytest.skip("a device is present: the pointers below are host placeholders")
+    lib = vrt.lib()
+    host = C.create_string_buffer(64)
+    p = C.addressof(host)
+    fake_world = C.c_void_p(p)  # never dereferenced on the rejection path
+    # render backward: at most 2^32 - 256 pixels
+    assert lib.vrt_render_bwd(fake_world, 0xFFFFFFFF, p, p, p, p, None) == L.VRT_ERR_INVALID_ARGUMENT
+    assert b"2^32 - 256" in lib.vrt_last_error()
+    # camera rays: 65536 x 65536 pixels is 2^32
+    cam = L.Camera()
+    cam.width = cam.height = 65536
+    cam.tile_w = cam.tile_h = 1
+    bufs = L.RayBuffers(p, p, p, p, p, p)
+    assert lib.vrt_camera_rays(C.byref(cam), C.byref(bufs), None) == L.VRT_ERR_INVALID_ARGUMENT
+    assert b"too many pixels" in lib.vrt_last_error()
+    # shader cast: at most 2^32 - 64 rays
+    rays = L.Rays(p, p, p, p, p, p)
+    rays.n = 0xFFFFFFF0
+    hits = L.Hits(p, (p + 15) & ~15, 0)
+    opts = L.ShaderOpts(0, 0, 0, 0)
+    assert lib.vrt_cast_shader(fake_world, C.byref(rays), C.byref(opts), C.byref(hits), None) == \
+        L.VRT_ERR_INVALID_ARGUMENT
+    assert b"too many rays" in lib.vrt_last_error()

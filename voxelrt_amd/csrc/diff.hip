@@ -165,7 +165,9 @@ int vrt_render_fwd(const vrt_world* world, const vrt_camera* cameras, uint32_t n
                 return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_render_fwd: camera size must be a multiple of its tile");
             total += (uint64_t)c.width * c.height;
         }
-        if (total > 0xFFFFFFFFull) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_render_fwd: more than 2^32 - 1 pixels");
+        // each launch's 64-lane blocks (at most one partial block per view) and every
+        // output pixel index must fit in 32 bits
+        if (total > 0xFFFFF000ull) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_render_fwd: more than 2^32 - 4096 pixels");
         if (total && (!albedo || !image || !saved_voxel || !saved_weight))
             return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_render_fwd: null argument");
         uint32_t base = 0;
@@ -179,7 +181,7 @@ int vrt_render_fwd(const vrt_world* world, const vrt_camera* cameras, uint32_t n
                 v.block_begin[k] = (uint32_t)blocks;
                 v.pixel_base[k] = base;
                 uint32_t n = c.width * c.height;
-                blocks += (n + 63) / 64;
+                blocks += ((uint64_t)n + 63) / 64;
                 base += n;
             }
             v.block_begin[v.nv] = (uint32_t)blocks;
@@ -197,6 +199,7 @@ int vrt_render_bwd(const vrt_world* world, uint32_t npixels, const float* grad_i
         if (!world || (npixels && (!grad_image || !saved_voxel || !saved_weight || !grad_albedo)))
             return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_render_bwd: null argument");
         if (npixels == 0) return VRT_OK;
+        if (npixels > 0xFFFFFF00u) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_render_bwd: more than 2^32 - 256 pixels");
         render_bwd_kernel<<<(npixels + 255) / 256, 256, 0, (hipStream_t)stream>>>(npixels, grad_image, saved_voxel,
                                                                                   saved_weight, grad_albedo);
         VRT_HIP(hipGetLastError());

@@ -245,7 +245,8 @@ int vrt_camera_rays(const vrt_camera* cam, vrt_ray_buffers* out, void* stream) {
         if (!out->ox || !out->oy || !out->oz || !out->dx || !out->dy || !out->dz)
             return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_camera_rays: null ray buffer");
         uint64_t n64 = (uint64_t)cam->width * cam->height;
-        if (n64 > 0xFFFFFFFFull) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_camera_rays: too many pixels");
+        // 256-thread blocks: the grid's lanes (blocks * 256) must fit in 32 bits
+        if (n64 > 0xFFFFFF00ull) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_camera_rays: too many pixels");
         uint32_t n = (uint32_t)n64;
         camera_rays_kernel<<<(n + 255) / 256, 256, 0, (hipStream_t)stream>>>(
             *cam, 1.0f / (float)cam->width, 1.0f / (float)cam->height, n, out->ox, out->oy, out->oz, out->dx, out->dy,

@@ -48,6 +48,8 @@ extern "C" int vrt_cast_shader(const vrt_world* world, const vrt_rays* rays, con
         int rc = check_rays_hits(rays, hits, r, h);
         if (rc != VRT_OK) return rc;
         if (r.n == 0) return VRT_OK;
+        // 64-lane blocks: the grid's lanes (blocks * 64) must fit in 32 bits
+        if (r.n > 0xFFFFFFC0u) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_cast_shader: too many rays");
         bool coarse = opts->method == VRT_SHADER_COARSE;
         uint32_t cap = opts->max_iters ? opts->max_iters : (coarse ? 96u : 256u);
         ShaderView sv = world->shader_view();
