@@ -18,7 +18,8 @@ __global__ __launch_bounds__(256) void shader_cast_kernel(ShaderView sv, RaysIn 
     const float d[3] = {r.dx[i], r.dy[i], r.dz[i]};
     const int32_t W[3] = {r.W[0], r.W[1], r.W[2]};
     ShaderHit sh;
-    bool hit = shader_trace<COARSE, LUT>(sv, o, d, W, cap, sh);
+    // one trace per kernel: the tame loop pays for its second loop copy in both methods
+    bool hit = shader_trace<COARSE, LUT, true>(sv, o, d, W, cap, sh);
     h.t[i] = sh.t;
     h.rec[i] = make_int4(sh.v[0], sh.v[1], sh.v[2], (int)(sh.mat | sh.code << 8 | (sh.iters & 0xFFFFu) << 16));
     if (h.pos_x) {

@@ -142,8 +142,9 @@ __device__ __forceinline__ bool shader_loop(const ShaderView& sv, const float d[
     return hit;
 }
 
-// rayCast (COARSE = false, :162-204) or rayCastCoarse (COARSE = true, :206-243).
-template <bool COARSE, bool LUT>
+// rayCast (COARSE = false, :162-204) or rayCastCoarse (COARSE = true, :206-243).  FAST
+// takes the tame loop when every ray of the wave is tame.
+template <bool COARSE, bool LUT, bool FAST = !COARSE>
 __device__ __forceinline__ bool shader_trace(const ShaderView& sv, const float o[3], const float d[3],
                                              const int32_t W[3], uint32_t cap, ShaderHit& out) {
     const WorldView& w = sv.w;
@@ -165,10 +166,11 @@ __device__ __forceinline__ bool shader_trace(const ShaderView& sv, const float o
     const uint32_t oct = (d[0] < 0.0f ? 0u : 1u) + (d[1] < 0.0f ? 0u : 2u) + (d[2] < 0.0f ? 0u : 4u);
     float tmin = 0.0f;
     uint32_t it = 0;
-    // the fast loop for rayCast; rayCastCoarse's short traces (cap 96, mostly shadow and bounce
-    // rays inside the renderer) measured no gain from it, and its second loop copy slowed
-    // the renderer's kernel, which holds three traces
-    bool hit = !COARSE && __all(tame)
+    // the fast loop by default for rayCast only: inside the renderer rayCastCoarse's short
+    // traces (cap 96, shadow and bounce rays) measured no gain from it, and its second loop
+    // copy slowed the renderer's kernel, which holds three traces; the batch kernel, which
+    // holds one, takes it for both (coarse C2 camera rays: 14.8 against 13.8 Grays/s)
+    bool hit = FAST && __all(tame)
                    ? shader_loop<COARSE, LUT, true>(sv, d, org, inv, ts, W, dm, oct, cap, v, side, cur, tmin, it)
                    : shader_loop<COARSE, LUT, false>(sv, d, org, inv, ts, W, dm, oct, cap, v, side, cur, tmin, it);
     // the record is built in locals and stored once, so the result stays in registers
