@@ -82,3 +82,24 @@ def test_clipped_and_special_rays(vrt, oracle_mod, cuda):
         for method, lut in MODES:
             for cap in (0, 7):
                 check_mode(world, ow, oo, d, W, method, lut, f"clip W={W} cap={cap}", cap)
+
+
+@pytest.mark.parametrize("size,ysize,seed", [(128, 64, 41), (64, 128, 42), (256, 32, 43)])
+def test_non_cubic_world_shader_parity(vrt, oracle_mod, cuda, size, ysize, seed):
+    """Worlds whose y extent differs from x/z: the two-compare bounds test, clipping to a
+    non-cubic box and sector-of-sectors groups that the world's y extent cuts short
+    (a 32-voxel-high world has one sector layer in groups four sectors high)."""
+    from helpers import random_rays as rr
+
+    vox = blob_world(size, seed=seed, density=0.004, blobs=5, ysize=ysize)
+    world = vrt.World.from_scene(vrt.Scene.from_dense(vox))
+    ow = dense_world(oracle_mod, vox)
+    info = world.info()
+    assert (info.size_xz, info.size_y) == (size, ysize)
+    for inside in (True, False):
+        o, d = rr(8000, size, seed=seed + 100, ysize=ysize, inside=inside)
+        for W in [(0, 0, 0), (32, -64, 96)]:
+            oo = (o - np.array(W, np.float32)[:, None]).astype(np.float32)
+            for method, lut in MODES:
+                rec = check_mode(world, ow, oo, d, W, method, lut, f"{size}x{ysize} inside={inside} W={W}")
+                assert ((rec[:, 3] & 0xFF) != 0).mean() > 0.05
