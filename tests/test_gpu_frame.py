@@ -13,7 +13,9 @@ pytestmark = pytest.mark.gpu
 SCHEDS = [(1, 0, 0, 0), (1, 0, 0, 1), (2, 16, 256, 0), (2, 8, 64, 0), (2, 32, 1024, 0), (2, 64, 128, 0)]
 
 
-@pytest.mark.parametrize("config", ["c1", "c2"])
+# c2 at each camera anchor: the sector corner, floor(eye) and the world's own origin take the
+# aligned, general and zero-origin specializations of the traversal
+@pytest.mark.parametrize("config", ["c1", "c2", "c2-floor", "c2-zero"])
 def test_frame_equals_separate_pipeline(vrt, cuda, config):
     import voxelrt_amd as V
     from voxelrt_amd import _lib as L
@@ -22,7 +24,8 @@ def test_frame_equals_separate_pipeline(vrt, cuda, config):
     if config == "c1":
         scene, cam = W.C1_WORLD.build(), W.c1_camera()
     else:
-        scene, cam = W.C2_WORLD.build(), W.c2_camera()
+        anchor = config.split("-")[1] if "-" in config else "sector"
+        scene, cam = W.C2_WORLD.build(), W.c2_camera(anchor=anchor)
     world = V.World.from_scene(scene)
     prim = cam.rays_device()
     hp_ref = world.cast(prim, W.MAX_ITERS, with_pos=True)
