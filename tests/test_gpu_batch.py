@@ -172,3 +172,31 @@ def test_autotune_picks_a_valid_tuning_and_caches(vrt, cuda, c2_small, tmp_path)
     assert ms > 0 and cache.exists() and len(cache.read_text().splitlines()) == 1
     best2, ms2 = V.autotune(world, batch, str(cache))
     assert best2 == best and ms2 == pytest.approx(ms)
+
+
+def test_segments_with_mixed_world_origins(vrt, cuda, c2_small):
+    """Segments whose world origins take the zero, general and aligned specializations of the
+    traversal, side by side in one launch, each equal to a plain cast of the same rays."""
+    import voxelrt_amd as V
+    from voxelrt_amd import workloads as W
+
+    world, _, cams, prim, segs, rays = c2_small
+    origins = [(0, 0, 0), (-7, 3, 11), (32, -64, 96)]
+    parts, msegs = [], []
+    off = 0
+    for j, (ro, n, W_, _) in enumerate(segs):
+        Wn = origins[j % len(origins)]
+        shift = torch.tensor([W_[a] - Wn[a] for a in range(3)], dtype=torch.float32, device="cuda")[:, None]
+        parts.append(rays.o[:, ro:ro + n] + shift)
+        msegs.append((off, n, Wn, None))
+        off += n
+    mrays = V.RayBatch(torch.cat(parts, 1).contiguous(), rays.d.contiguous(), (0, 0, 0))
+    ref = plain_results(world, cams, prim, msegs, mrays, W.MAX_ITERS)
+    for tuning in ((16, 4, 1, 0), (8, 8, 2, 0)):
+        batch = V.Batch(cams, msegs, mrays, W.MAX_ITERS)
+        V.BatchPlan(world, batch, tuning).launch()
+        torch.cuda.synchronize()
+        for k in range(len(prim) + len(msegs)):
+            t, rec, _ = batch.segment_hits(k)
+            assert np.array_equal(rec.cpu().numpy(), ref[k][1]), f"segment {k} tuning {tuning}"
+            assert np.array_equal(t.cpu().numpy().view(np.uint32), ref[k][0].view(np.uint32)), f"segment {k}"
