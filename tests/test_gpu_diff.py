@@ -24,11 +24,15 @@ def c5_small():
     return world, scene, cams, albedo, nvox
 
 
-def test_forward_matches_oracle(vrt, oracle_mod, cuda, c5_small):
+# the camera anchor picks the traversal's aligned (sector), general (floor) or zero-origin form
+@pytest.mark.parametrize("anchor", ["sector", "floor", "zero"])
+def test_forward_matches_oracle(vrt, oracle_mod, cuda, c5_small, anchor):
     import voxelrt_amd as V
     from voxelrt_amd import workloads as W
 
     world, scene, cams, albedo, nvox = c5_small
+    if anchor != "sector":
+        cams = W.c5_cameras(n=4, size=256, anchor=anchor)
     image, sv, sw = V.render_saved(world, albedo, cams, W.MAX_ITERS)
     torch.cuda.synchronize()
     image, sv, sw = image.cpu().numpy(), sv.cpu().numpy().astype(np.int64), sw.cpu().numpy()

@@ -89,13 +89,13 @@ def c4_cameras(n: int = 64, seed: int = 4, tile=(8, 8)) -> list[CameraSpec]:
     return cams
 
 
-def c5_cameras(n: int = 16, size: int = 512, tile=(8, 8)) -> list[CameraSpec]:
+def c5_cameras(n: int = 16, size: int = 512, tile=(8, 8), anchor: str = "sector") -> list[CameraSpec]:
     cams = []
     for k in range(n):
         a = 2 * math.pi * k / n
         pos = (256 + 200 * math.sin(a) + 0.31, 420.57, 256 + 200 * math.cos(a) + 0.17)
         yaw = math.degrees(a) + 180.0
-        cams.append(CameraSpec(pos, yaw, -35.0, size, size, tile=tile, anchor="sector"))
+        cams.append(CameraSpec(pos, yaw, -35.0, size, size, tile=tile, anchor=anchor))
     return cams
 
 
