@@ -69,6 +69,13 @@ void oracle_cast(const oracle_world* w, const float* const o[3], const float* co
                  const int32_t world_origin[3], uint32_t begin, uint32_t end, uint32_t max_iters,
                  float* t, oracle_hit* rec, float* const pos[3], uint16_t* descents);
 
+/* GetStepPos (CpuRenderer.cpp:135-171) alone, for n in-bounds voxel positions v_in[3 * i]
+ * and directions d[3 * i]: solid[i] = 1 when the voxel is solid (v_out = v_in), else v_out
+ * is the far corner, in the ray direction, of the empty cell the step skips; descended[i] = 1
+ * when the step read the brick's occupancy (the brick is allocated). */
+void oracle_step_pos(const oracle_world* w, uint32_t n, const int32_t* v_in, const float* d, int32_t* v_out,
+                     uint8_t* solid, uint8_t* descended);
+
 /* Counter totals of a scalar cast (same definitions as the product's counters). */
 void oracle_count(const oracle_world* w, const float* const o[3], const float* const d[3],
                   const int32_t world_origin[3], uint32_t n, uint32_t max_iters,
@@ -76,11 +83,12 @@ void oracle_count(const oracle_world* w, const float* const o[3], const float* c
 
 /* SIMD restatement of the reference's vector loop (CpuRenderer.cpp:172-224):
  * 16 lanes on AVX-512, else 8 lanes on AVX2, consecutive rays per vector,
- * `threads` worker threads (0 = all cores).  Results identical to oracle_cast.
+ * `threads` worker threads (0 = all cores).  Results identical to oracle_cast
+ * (pos, optional: the hit position relative to world_origin, 0 on a miss).
  * Returns the lane width used. */
 int oracle_cast_simd(const oracle_world* w, const float* const o[3], const float* const d[3],
                      const int32_t world_origin[3], uint32_t n, uint32_t max_iters,
-                     uint32_t threads, float* t, oracle_hit* rec);
+                     uint32_t threads, float* t, oracle_hit* rec, float* const pos[3]);
 /* Lane width oracle_cast_simd would use on this host (16, 8, or 0 if unsupported). */
 int oracle_simd_width(void);
 int oracle_num_cores(void);

@@ -45,7 +45,8 @@ def lib() -> C.CDLL:
         h.oracle_count.argtypes = [P, C.POINTER(P * 3), C.POINTER(P * 3), C.POINTER(C.c_int32 * 3), u32, u32,
                                    C.POINTER(OracleCounters)]
         h.oracle_cast_simd.argtypes = [P, C.POINTER(P * 3), C.POINTER(P * 3), C.POINTER(C.c_int32 * 3), u32, u32, u32,
-                                       P, P]
+                                       P, P, C.POINTER(P * 3)]
+        h.oracle_step_pos.argtypes = [P, u32, P, P, P, P, P]
         h.oracle_cast_simd.restype = C.c_int
         h.oracle_simd_width.restype = C.c_int
         h.oracle_num_cores.restype = C.c_int
@@ -129,18 +130,31 @@ class OracleWorld:
         lib().oracle_count(self._h, C.byref(_p3(o)), C.byref(_p3(d)), C.byref(W), o.shape[1], max_iters, C.byref(c))
         return {"rays": c.rays, "iters": c.iters, "descents": c.descents, "hits": c.hits}
 
-    def cast_simd(self, o, d, world_origin=(0, 0, 0), max_iters=128, threads=0):
+    def cast_simd(self, o, d, world_origin=(0, 0, 0), max_iters=128, threads=0, with_pos=False):
         o = np.ascontiguousarray(o, dtype=np.float32)
         d = np.ascontiguousarray(d, dtype=np.float32)
         n = o.shape[1]
         t = np.zeros(n, dtype=np.float32)
         rec = np.zeros((n, 4), dtype=np.int32)
+        pos = np.zeros((3, n), dtype=np.float32) if with_pos else None
         W = (C.c_int32 * 3)(*world_origin)
         lanes = lib().oracle_cast_simd(self._h, C.byref(_p3(o)), C.byref(_p3(d)), C.byref(W), n, max_iters, threads,
-                                       t.ctypes.data, rec.ctypes.data)
+                                       t.ctypes.data, rec.ctypes.data, C.byref(_p3(pos)) if with_pos else None)
         if lanes == 0:
             raise RuntimeError("host has neither AVX-512 nor AVX2+FMA")
-        return {"t": t, "rec": rec, "lanes": lanes}
+        return {"t": t, "rec": rec, "pos": pos, "lanes": lanes}
+
+    def step_pos(self, v: np.ndarray, d: np.ndarray):
+        """GetStepPos alone for in-bounds voxels v [n, 3] and directions d [n, 3]: (v_out, solid, descended)."""
+        v = np.ascontiguousarray(v, dtype=np.int32)
+        d = np.ascontiguousarray(d, dtype=np.float32)
+        n = v.shape[0]
+        out = np.zeros((n, 3), np.int32)
+        solid = np.zeros(n, np.uint8)
+        desc = np.zeros(n, np.uint8)
+        lib().oracle_step_pos(self._h, n, v.ctypes.data, d.ctypes.data, out.ctypes.data, solid.ctypes.data,
+                              desc.ctypes.data)
+        return out, solid.astype(bool), desc.astype(bool)
 
     def cast_shader(self, o, d, world_origin=(0, 0, 0), method="full", lut=False, max_iters=0, begin=0, end=None,
                     with_pos=False) -> dict:

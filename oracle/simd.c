@@ -188,6 +188,7 @@ typedef struct job {
     uint32_t n, max_iters, lanes;
     float* t;
     oracle_hit* rec;
+    float* const* pos;
     atomic_uint next;   /* next block of 64 vectors to claim */
 } job;
 
@@ -202,9 +203,9 @@ static void* worker(void* arg) {
         for (uint32_t base = (uint32_t)start; base < end; base += j->lanes) {
             uint32_t cnt = end - base < j->lanes ? end - base : j->lanes;
             if (j->lanes == 16)
-                cast_block_avx512(j->w, j->o, j->d, j->W, base, cnt, j->max_iters, j->t, j->rec);
+                cast_block_avx512(j->w, j->o, j->d, j->W, base, cnt, j->max_iters, j->t, j->rec, j->pos);
             else
-                cast_block_avx2(j->w, j->o, j->d, j->W, base, cnt, j->max_iters, j->t, j->rec);
+                cast_block_avx2(j->w, j->o, j->d, j->W, base, cnt, j->max_iters, j->t, j->rec, j->pos);
         }
     }
     return NULL;
@@ -212,7 +213,7 @@ static void* worker(void* arg) {
 
 int oracle_cast_simd(const oracle_world* w, const float* const o[3], const float* const d[3],
                      const int32_t world_origin[3], uint32_t n, uint32_t max_iters,
-                     uint32_t threads, float* t, oracle_hit* rec) {
+                     uint32_t threads, float* t, oracle_hit* rec, float* const pos[3]) {
     int lanes = oracle_simd_width();
     if (lanes == 0) return 0;
     if (threads == 0) threads = (uint32_t)oracle_num_cores();
@@ -226,6 +227,7 @@ int oracle_cast_simd(const oracle_world* w, const float* const o[3], const float
     j.lanes = (uint32_t)lanes;
     j.t = t;
     j.rec = rec;
+    j.pos = pos;
     atomic_init(&j.next, 0);
     if (threads == 1) {
         worker(&j);

@@ -232,6 +232,19 @@ void oracle_cast(const oracle_world* w, const float* const o[3], const float* co
     }
 }
 
+void oracle_step_pos(const oracle_world* w, uint32_t n, const int32_t* v_in, const float* d, int32_t* v_out,
+                     uint8_t* solid, uint8_t* descended) {
+    for (uint32_t i = 0; i < n; i++) {
+        int32_t v[3] = {v_in[3 * i], v_in[3 * i + 1], v_in[3 * i + 2]};
+        uint32_t desc = 0;
+        solid[i] = (uint8_t)get_step_pos(w, v, d + 3 * i, &desc);
+        descended[i] = (uint8_t)desc;
+        v_out[3 * i] = v[0];
+        v_out[3 * i + 1] = v[1];
+        v_out[3 * i + 2] = v[2];
+    }
+}
+
 void oracle_count(const oracle_world* w, const float* const o[3], const float* const d[3],
                   const int32_t world_origin[3], uint32_t n, uint32_t max_iters, oracle_counters* out) {
     memset(out, 0, sizeof(*out));
