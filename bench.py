@@ -295,7 +295,7 @@ def main() -> int:
 
     def opts(counters=None, cnt=None, which="primary"):
         return L.CastOpts(sched, args.refill, args.chunk, 0, counters.data_ptr() if counters is not None else 0,
-                          cnt.data_ptr() if cnt is not None else 0, 0, 0, 0, 0, order[which])
+                          cnt.data_ptr() if cnt is not None else 0, order[which], 0)
 
     cur = {"primary": opts(), "bounce": opts(cnt=count, which="bounce")}  # the timed launches' options
 

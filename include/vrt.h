@@ -48,7 +48,7 @@
 extern "C" {
 #endif
 
-#define VRT_ABI_VERSION 1
+#define VRT_ABI_VERSION 2
 
 /* ---------------------------------------------------------------- errors */
 
@@ -86,6 +86,11 @@ typedef struct vrt_world_info {
     uint64_t device_bytes;         /* HBM held by the world buffers */
     uint32_t slot_high_water;      /* one past the highest allocated slot: per-voxel arrays in
                                       slot order (the render's albedo) need this many * 512 entries */
+    uint32_t generation;           /* changes whenever the occupancy and voxel buffers move (slot
+                                      growth): device pointers taken before are then stale */
+    uint64_t upload_bytes;         /* host-to-device bytes sent by syncs since creation */
+    uint64_t upload_brick_bytes;   /* of those, brick bytes: 512 per uploaded brick */
+    uint64_t moved_bricks;         /* bricks moved between slots on the device (contents kept) */
 } vrt_world_info;
 
 int vrt_world_create(const vrt_world_desc* desc, void* stream, vrt_world** out);
@@ -93,18 +98,38 @@ int vrt_world_destroy(vrt_world* world);
 int vrt_world_get_info(const vrt_world* world, vrt_world_info* out);
 
 /*
- * Replace the contents of a set of sectors (the DirtyLocs protocol of
- * GpuRenderer.cpp:45-167).  For each i < nsec:
+ * Update a set of sectors: the DirtyLocs protocol of GpuRenderer.cpp:45-167, where
+ * the caller records DirtyLocs[sectorIdx] |= brickMask (VoxelMap.h:185) and the
+ * renderer's SyncBuffers consumes it.  For each i < nsec:
  *   sector_idx[i]  linear sector index x | z << sx | y << 2sx (sx = log2 sectors_xz)
  *   alloc_mask[i]  bricks present in the sector after the update (0 = sector emptied)
- * bricks holds, sector after sector in the order given, popcount(alloc_mask[i])
- * bricks of 512 bytes each in increasing brick index; brick voxel order is
- * x | z << 3 | y << 6.  Slots are (re)allocated on the host, brick bytes are
- * copied to HBM and the occupancy masks of every uploaded brick are rebuilt on
- * the device.  Host pointers; the call returns once the host buffers may be
- * reused (uploads are staged), the device work is ordered on `stream`.
- * Bricks that are entirely empty may be listed; they stay allocated.
+ *   dirty_mask[i]  bricks whose bytes follow in `bricks` (only its bits inside
+ *                  alloc_mask count, as dirtyMask &= allocMask, GpuRenderer.cpp:62);
+ *                  a brick newly set in alloc_mask must be dirty
+ * bricks holds, sector after sector in the order given, the dirty bricks of 512
+ * bytes each in increasing brick index; brick voxel order is x | z << 3 | y << 6.
+ * Slots are (re)allocated on the host; only dirty bricks are uploaded and have
+ * their occupancy masks rebuilt on the device.  Bricks that stay allocated and
+ * are not dirty keep their device contents: when the sector's slot run moves or
+ * their slot shifts with the allocation, they are moved on the device (the
+ * reference re-uploads the whole sector then, BrickSlotAllocator.cpp:7-23).
+ * Uploads go through staging buffers the world owns (8 MiB pinned halves, kept
+ * from the first sync), so the call allocates nothing after its first use except
+ * when slots or moves outgrow their buffers, and does not wait for the device
+ * unless a staging half is still in use.  Host pointers; the call returns once
+ * the host buffers may be reused, the device work is ordered on `stream`, and
+ * casts on other streams must not overlap it.  On VRT_ERR_CAPACITY or
+ * VRT_ERR_INVALID_ARGUMENT the world is unchanged; a HIP failure once device
+ * work has begun leaves the world failed (every later call on it returns
+ * VRT_ERR_STATE).  Bricks that are entirely empty may be listed; they stay
+ * allocated.
  */
+int vrt_world_sync_bricks(vrt_world* world, uint32_t nsec, const uint32_t* sector_idx,
+                          const uint64_t* alloc_mask, const uint64_t* dirty_mask,
+                          const uint8_t* bricks, void* stream);
+
+/* vrt_world_sync_bricks with every brick of each sector dirty: bricks holds
+ * popcount(alloc_mask[i]) bricks per sector (MarkAllDirty, VoxelMap.h:207-211). */
 int vrt_world_sync(vrt_world* world, uint32_t nsec, const uint32_t* sector_idx,
                    const uint64_t* alloc_mask, const uint8_t* bricks, void* stream);
 
@@ -177,17 +202,11 @@ int vrt_cast(const vrt_world* world, const vrt_rays* rays, uint32_t max_iters,
 
 /* Kernel variants of the same cast (results are identical for every variant). */
 typedef enum vrt_cast_schedule {
-    VRT_SCHED_AUTO = 0,        /* pick from the autotuner cache, else the default */
+    VRT_SCHED_AUTO = 0,        /* the default (GRID) */
     VRT_SCHED_GRID = 1,        /* one ray per lane, one lane per ray, grid covers n */
-    VRT_SCHED_PERSISTENT = 2,  /* persistent waves pull ray chunks from a queue and refill idle lanes */
-    VRT_SCHED_MULTIPASS = 3,   /* passes of pass_iters lookups; rays still running are packed into
-                                  dense waves again for the next pass (long-tailed ray sets) */
-    VRT_SCHED_REPACK = 4,      /* workgroups of 256 rays; every pass_iters lookups the rays still running
-                                  are packed into the workgroup's first waves through LDS, so waves stay
-                                  dense (long-tailed ray sets such as diffuse bounces) */
-    VRT_SCHED_LANES = 5        /* each lane traces `chunk` rays (1..8, 0 = 4) one after another, the
-                                  waiting ones staged in LDS, so a lane whose ray ends starts the next
-                                  (long-tailed ray sets) */
+    VRT_SCHED_PERSISTENT = 2   /* persistent waves pull ray chunks from a queue and refill idle lanes;
+                                  the queue counters belong to the launch's (device, stream) pair, so
+                                  persistent casts on different streams never share them */
 } vrt_cast_schedule;
 
 /* Order in which the grid schedule dispatches its blocks of 64 rays (results are identical). */
@@ -200,26 +219,17 @@ typedef enum vrt_cast_order {
 typedef struct vrt_cast_opts {
     uint32_t schedule;          /* vrt_cast_schedule */
     uint32_t refill;            /* persistent: free lanes that trigger a retire/refill, 1..64 (0 = 16) */
-    uint32_t chunk;             /* persistent: rays per queue grab, multiple of 64 (0 = default);
-                                   lanes: rays per lane, 1..8 (0 = 4) */
+    uint32_t chunk;             /* persistent: rays per queue grab, multiple of 64 (0 = 64) */
     uint32_t waves_per_cu;      /* persistent occupancy target (0 = default) */
     vrt_cast_counters* counters;/* optional device pointer, accumulated (not cleared) */
     const uint32_t* count;      /* optional device pointer: rays to trace, read when the kernel
                                    runs (rays->n is then the capacity and the upper bound) */
-    uint32_t pass_iters;        /* multipass: lookups per pass; repack: lookups between repacks (0 = 16) */
-    uint32_t passes;            /* multipass: number of passes, the last one runs to max_iters (0 = 5) */
-    void* scratch;              /* multipass: device scratch of vrt_cast_scratch_bytes(n) bytes; NULL = a
-                                   per-device buffer owned by the library (multipass casts on one device
-                                   must then not overlap across streams) */
-    size_t scratch_bytes;
     uint32_t order;             /* grid: vrt_cast_order (0 = forward) */
+    uint32_t reserved;          /* 0 */
 } vrt_cast_opts;
 
 int vrt_cast_ex(const vrt_world* world, const vrt_rays* rays, uint32_t max_iters,
                 vrt_hits* hits, const vrt_cast_opts* opts, void* stream);
-
-/* Bytes of device scratch a multipass cast of n rays needs (continuation records and counters). */
-size_t vrt_cast_scratch_bytes(uint32_t n);
 
 /*
  * Scalar FP64 cast of one ray on the device, the semantics of VoxelMap::RayCast

@@ -87,6 +87,12 @@ public:
               void* stream = nullptr) {
         check(vrt_world_sync(w_, nsec, sector_idx, alloc_mask, bricks, stream));
     }
+    // The sectors of one DirtyLocs batch with their dirty masks: only the dirty bricks
+    // (dirty_mask & alloc_mask) follow in `bricks`; the rest keep their device contents.
+    void SyncBricks(uint32_t nsec, const uint32_t* sector_idx, const uint64_t* alloc_mask, const uint64_t* dirty_mask,
+                    const uint8_t* bricks, void* stream = nullptr) {
+        check(vrt_world_sync_bricks(w_, nsec, sector_idx, alloc_mask, dirty_mask, bricks, stream));
+    }
     // Material::GetEncoded of the 256 palette entries (VoxelMap.h:27-41).
     void SetPalette(const uint64_t palette[256], void* stream = nullptr) {
         check(vrt_world_set_palette(w_, palette, stream));

@@ -129,3 +129,25 @@ def test_round_trip_and_errors(vrt, tmp_path):
         vrt.Scene.load_cvox(str(bad), 7, 7)
     with pytest.raises(vrt.VrtError, match="cannot open"):
         vrt.Scene.load_cvox(str(tmp_path / "missing.cvox"), 7, 7)
+
+
+def test_committed_fixture_loads_to_its_grid(vrt):
+    """tests/golden/terrain128.cvox (written by tests/golden/make_cvox.py) loads to exactly the
+    dense grid it was written from, with the palette of its material records."""
+    from pathlib import Path
+
+    g = Path(__file__).resolve().parent / "golden"
+    f = np.load(g / "terrain128.npz")
+    sc, info = vrt.Scene.load_cvox(str(g / "terrain128.cvox"), 7, 7,
+                                   sector_offset=tuple(int(x) for x in f["sector_offset"]))
+    ref = vrt.Scene.from_dense(f["voxels"])
+    a, b = sc.stream(), ref.stream()
+    for k in ("sector_idx", "alloc_mask", "bricks"):
+        assert np.array_equal(a[k], b[k]), k
+    mats = f["materials"]
+    em = mats[:, 4:8].copy().view(np.float32).ravel()
+    with np.errstate(over="ignore"):
+        half = em.astype(np.float16).view(np.uint16).astype(np.uint64)
+    want = ((mats[:, 0].astype(np.uint64) >> 3) << 11) | ((mats[:, 1].astype(np.uint64) >> 2) << 5) | \
+        (mats[:, 2].astype(np.uint64) >> 3) | (half << 16) | (mats[:, 3].astype(np.uint64) << 32)
+    assert np.array_equal(sc.palette(), want)

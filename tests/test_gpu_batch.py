@@ -200,3 +200,119 @@ def test_segments_with_mixed_world_origins(vrt, cuda, c2_small):
             t, rec, _ = batch.segment_hits(k)
             assert np.array_equal(rec.cpu().numpy(), ref[k][1]), f"segment {k} tuning {tuning}"
             assert np.array_equal(t.cpu().numpy().view(np.uint32), ref[k][0].view(np.uint32)), f"segment {k}"
+
+
+def test_graph_recaptured_after_slot_growth(vrt, oracle_mod, cuda):
+    """A sync that outgrows the slot buffers moves them (the world's generation changes): the next
+    graph replay captures again with the new buffers and equals an eager launch and the oracle."""
+    import voxelrt_amd as V
+    from helpers import blob_world
+    from test_gpu_world import sector_stream_of
+    from test_oracle import dense_world
+
+    vox = blob_world(128, seed=5, density=0.002)
+    world = V.World(7, 7)
+    world.sync(*sector_stream_of(vox, list(range(64))))
+    i0 = world.info()
+    cams = [V.CameraSpec((64.5, 100.25, 10.75), 20.0, -35.0, 64, 64),
+            V.CameraSpec((20.5, 90.5, 100.5), 200.0, -50.0, 128, 64, anchor="sector")]
+    batch = V.Batch(cams, max_iters=128)
+    plan = V.BatchPlan(world, batch, (8, 8, 1, 0))
+    plan.graph_launch()
+    torch.cuda.synchronize()
+    # fill the lower half of the world: many more bricks than the slot buffers hold
+    vox2 = vox.copy()
+    vox2[:60] = np.where(vox2[:60] == 0, 4, vox2[:60])
+    world.sync(*sector_stream_of(vox2, list(range(64))))
+    i1 = world.info()
+    assert i1.slot_capacity > i0.slot_capacity and i1.generation != i0.generation
+    ow = dense_world(oracle_mod, vox2)
+    for rep in range(2):
+        batch.hits.rec.fill_(-3)
+        torch.cuda.synchronize()
+        plan.graph_launch()
+        torch.cuda.synchronize()
+        g_t, g_rec = batch.hits.t.clone(), batch.hits.rec.clone()
+        plan.launch()
+        torch.cuda.synchronize()
+        assert torch.equal(g_rec, batch.hits.rec) and torch.equal(g_t, batch.hits.t), f"replay {rep}"
+        for k, c in enumerate(cams):
+            o, d = c.rays_host()
+            ref = ow.cast(o, d, c.world_origin, 128)
+            t, rec, _ = batch.segment_hits(k)
+            assert_hits_equal(t.cpu().numpy(), rec.cpu().numpy(), ref["t"], ref["rec"], f"camera {k} replay {rep}")
+    plan.close()
+
+
+def test_plan_rejects_32bit_index_overflow(vrt, cuda, c2_small):
+    """Hit and ray indices are 32-bit in the kernel: segments reaching past index 2^32 - 1 and
+    cameras of more than 2^32 - 256 pixels are rejected when the plan is built."""
+    import ctypes as C
+
+    import voxelrt_amd as V
+    from voxelrt_amd import _lib as L
+
+    world, scene, cams, prim, segs, rays = c2_small
+    for hit_off, ray_off, cnt in [(0xFFFFFF00, 0, 0x200), (0, 0xFFFFFFF0, 64), (0, 0, 0xFFFFFFF0)]:
+        seg = L.Segment(L.VRT_SEG_RAYS, 0, ray_off, hit_off, cnt, (C.c_int32 * 3)(0, 0, 0), 0)
+        b = L.Batch(None, 0, C.cast(C.pointer(seg), C.c_void_p), 1, rays.to_c(), V.HitBatch.empty(64).to_c(), 128)
+        out = C.c_void_p()
+        t = L.BatchTuning(8, 8, 1, 0)
+        assert V.lib().vrt_batch_plan_create(world.handle, C.byref(b), C.byref(t), None, C.byref(out)) == \
+            L.VRT_ERR_INVALID_ARGUMENT
+        assert b"2^32" in V.lib().vrt_last_error()
+    big = L.Camera.from_buffer_copy(cams[0].c)
+    big.width, big.height, big.tile_w, big.tile_h = 65536, 65536, 1, 1
+    seg = L.Segment(L.VRT_SEG_CAMERA, 0, 0, 0, 0, (C.c_int32 * 3)(0, 0, 0), 0)
+    b = L.Batch(C.cast(C.pointer(big), C.c_void_p), 1, C.cast(C.pointer(seg), C.c_void_p), 1, L.Rays(),
+                V.HitBatch.empty(64).to_c(), 128)
+    out = C.c_void_p()
+    t = L.BatchTuning(8, 8, 1, 0)
+    assert V.lib().vrt_batch_plan_create(world.handle, C.byref(b), C.byref(t), None, C.byref(out)) == \
+        L.VRT_ERR_INVALID_ARGUMENT
+    assert b"2^32 - 256 pixels" in V.lib().vrt_last_error()
+
+
+def test_c4_full_batch_matches_oracle(vrt, oracle_mod, cuda, c2_small):
+    """The whole batched configuration: 64 cameras of uneven square sizes (128..1024 px) and each
+    camera's bounce set truncated to a seeded fraction U(0.25, 1), 128 segments traced in one
+    hipGraph launch, compared per pixel and per bounce ray with the oracle."""
+    import voxelrt_amd as V
+    from voxelrt_amd import workloads as W
+
+    world, scene = c2_small[0], c2_small[1]
+    cams = W.c4_cameras(64, seed=4, tile=(1, 1))
+    rng = np.random.default_rng(44)
+    parts, segs = [], []
+    off = 0
+    for c in cams:
+        rb = c.rays_device()
+        h = world.cast(rb, W.MAX_ITERS, with_pos=True)
+        br, _, cnt = V.bounce(rb, h, c.c.width, W.BOUNCE_SEED)
+        torch.cuda.synchronize()
+        nb = int(int(cnt.item()) * rng.uniform(0.25, 1.0))
+        parts.append((br.o[:, :nb].clone(), br.d[:, :nb].clone()))
+        segs.append((off, nb, rb.world_origin, None))
+        off += nb
+        del rb, h, br
+    rays = V.RayBatch(torch.cat([p[0] for p in parts], 1).contiguous(),
+                      torch.cat([p[1] for p in parts], 1).contiguous(), (0, 0, 0))
+    del parts
+    batch = V.Batch(cams, segs, rays, W.MAX_ITERS)
+    plan = V.BatchPlan(world, batch, (16, 4, 1, 0))
+    plan.graph_launch()
+    torch.cuda.synchronize()
+    n_prim = sum(c.n for c in cams)
+    print(f"C4: {n_prim} primary + {off} bounce rays in {len(cams) + len(segs)} segments", flush=True)
+    assert n_prim > 20_000_000
+    ow = oracle_mod.OracleWorld.from_stream(scene.stream())
+    for k, c in enumerate(cams):
+        o, d = c.rays_host()
+        r = ow.cast_simd(o, d, c.world_origin, W.MAX_ITERS)
+        t, rec, _ = batch.segment_hits(k)
+        assert_hits_equal(t.cpu().numpy(), rec.cpu().numpy(), r["t"], r["rec"], f"camera {k}")
+    for j, (ro, n, W_, _) in enumerate(segs):
+        r = ow.cast_simd(rays.o[:, ro:ro + n].cpu().numpy(), rays.d[:, ro:ro + n].cpu().numpy(), W_, W.MAX_ITERS)
+        t, rec, _ = batch.segment_hits(len(cams) + j)
+        assert_hits_equal(t.cpu().numpy(), rec.cpu().numpy(), r["t"], r["rec"], f"bounce set {j}")
+    plan.close()

@@ -13,25 +13,17 @@ from test_oracle import dense_world
 pytestmark = pytest.mark.gpu
 
 GOLDEN = Path(__file__).resolve().parent / "golden"
-# (schedule, chunk or lookups per pass): grid; persistent with 64/256/1024-ray chunks;
-# multipass with the default 16 lookups x 5 passes and with 4 lookups x 8 passes;
-# repack every 16 (default) and every 3 lookups; lanes of 4 (default), 1, 3 and 8 rays
-SCHEDULES = [(1, 0), (2, 64), (2, 256), (2, 1024), (3, 0), (3, 4), (4, 0), (4, 3), (5, 0), (5, 1), (5, 3), (5, 8)]
+# (schedule, chunk): grid in forward and reverse block order; persistent with 64/256/1024-ray chunks
+SCHEDULES = [(1, 0), (-1, 0), (2, 64), (2, 256), (2, 1024)]
 
 
 def gpu_cast(world, o, d, W, max_iters=128, schedule=1, chunk=0, with_pos=False, counters=None):
     from voxelrt_amd import RayBatch
 
     rb = RayBatch.from_numpy(o, d, W)
-    if schedule == 3:
-        h = world.cast(rb, max_iters, with_pos=with_pos, schedule=3, pass_iters=chunk, passes=8 if chunk else 0,
-                       counters=counters)
-    elif schedule == 4:
-        h = world.cast(rb, max_iters, with_pos=with_pos, schedule=4, pass_iters=chunk, counters=counters)
-    elif schedule == 5:
-        h = world.cast(rb, max_iters, with_pos=with_pos, schedule=5, chunk=chunk, counters=counters)
-    else:
-        h = world.cast(rb, max_iters, with_pos=with_pos, schedule=schedule, chunk=chunk, counters=counters)
+    # schedule -1: the grid with its blocks dispatched last first
+    h = world.cast(rb, max_iters, with_pos=with_pos, schedule=abs(schedule), chunk=chunk, counters=counters,
+                   order=1 if schedule < 0 else 0)
     torch.cuda.synchronize()
     return h.t.cpu().numpy(), h.rec.cpu().numpy(), (h.pos.cpu().numpy() if h.pos is not None else None)
 
@@ -173,7 +165,7 @@ def test_special_rays(vrt, oracle_mod, cuda):
     for v in (vox, full, empty):
         world, ow = pair_from_dense(vrt, oracle_mod, v)
         ref = ow.cast(o, d, (0, 0, 0), 128)
-        for sched, chunk in SCHEDULES[:2] + SCHEDULES[4:]:
+        for sched, chunk in SCHEDULES:
             t, rec, _ = gpu_cast(world, o, d, (0, 0, 0), 128, sched, chunk)
             assert_hits_equal(t, rec, ref["t"], ref["rec"], f"special sched={sched}")
             assert np.array_equal(t.view(np.uint32), ref["t"].view(np.uint32))

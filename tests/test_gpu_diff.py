@@ -24,15 +24,8 @@ def c5_small():
     return world, scene, cams, albedo, nvox
 
 
-# the camera anchor picks the traversal's aligned (sector), general (floor) or zero-origin form
-@pytest.mark.parametrize("anchor", ["sector", "floor", "zero"])
-def test_forward_matches_oracle(vrt, oracle_mod, cuda, c5_small, anchor):
-    import voxelrt_amd as V
-    from voxelrt_amd import workloads as W
-
-    world, scene, cams, albedo, nvox = c5_small
-    if anchor != "sector":
-        cams = W.c5_cameras(n=4, size=256, anchor=anchor)
+def check_forward(V, W, oracle_mod, world, scene, cams, albedo):
+    """Forward render against the oracle's cast and the Lambert weight, pixel by pixel."""
     image, sv, sw = V.render_saved(world, albedo, cams, W.MAX_ITERS)
     torch.cuda.synchronize()
     image, sv, sw = image.cpu().numpy(), sv.cpu().numpy().astype(np.int64), sw.cpu().numpy()
@@ -40,7 +33,7 @@ def test_forward_matches_oracle(vrt, oracle_mod, cuda, c5_small, anchor):
     L = np.array(V.sun_direction(), np.float32)
     alb = albedo.cpu().numpy()
     base = 0
-    seen = {}
+    keys, ids = [], []
     for cam in cams:
         o, d = cam.rays_host()
         r = ow.cast_simd(o, d, cam.world_origin, W.MAX_ITERS)
@@ -55,25 +48,44 @@ def test_forward_matches_oracle(vrt, oracle_mod, cuda, c5_small, anchor):
         wgt = np.maximum(ndl, np.float32(0)) * np.float32(5.0)
         assert np.array_equal(s_w[hit].view(np.uint32), wgt[hit].astype(np.float32).view(np.uint32))
         # the saved id is slot * 512 + voxel index in the brick, one id per voxel
-        vx, vy, vz = rec[hit, 0], rec[hit, 1], rec[hit, 2]
+        vx, vy, vz = rec[hit, 0].astype(np.int64), rec[hit, 1].astype(np.int64), rec[hit, 2].astype(np.int64)
         vi = (vx & 7) | (vz & 7) << 3 | (vy & 7) << 6
         assert np.array_equal(s_v[hit] & 511, vi)
-        for key, gv in zip(zip(vx, vy, vz), s_v[hit]):
-            assert seen.setdefault(key, gv) == gv
+        keys.append(vx | vy << 20 | vz << 40)
+        ids.append(s_v[hit])
         exp = alb[s_v[hit]] * s_w[hit][:, None]
         assert np.array_equal(image[base:base + n][hit].view(np.uint32), exp.astype(np.float32).view(np.uint32))
         assert np.all(image[base:base + n][~hit] == 0)
         base += n
-    assert len(set(seen.values())) == len(seen)
+    # voxel <-> id is one to one: as many distinct (voxel, id) pairs as distinct voxels and ids
+    keys, ids = np.concatenate(keys), np.concatenate(ids)
+    nk, ni = len(np.unique(keys)), len(np.unique(ids))
+    assert nk == ni == len(np.unique(np.stack([keys, ids], 1), axis=0))
+    return base
+
+
+# the camera anchor picks the traversal's aligned (sector), general (floor) or zero-origin form
+@pytest.mark.parametrize("anchor", ["sector", "floor", "zero"])
+def test_forward_matches_oracle(vrt, oracle_mod, cuda, c5_small, anchor):
+    import voxelrt_amd as V
+    from voxelrt_amd import workloads as W
+
+    world, scene, cams, albedo, nvox = c5_small
+    if anchor != "sector":
+        cams = W.c5_cameras(n=4, size=256, anchor=anchor)
+    check_forward(V, W, oracle_mod, world, scene, cams, albedo)
 
 
 def test_backward_matches_reference_and_finite_differences(vrt, oracle_mod, cuda, c5_small):
     """Tolerance 1e-4 relative to the sum of |terms| of each voxel's gradient: the scale of the
     rounding error of the fp32 atomics (and of the fp32 image the loss is taken on)."""
+    check_backward(oracle_mod, *c5_small)
+
+
+def check_backward(oracle_mod, world, scene, cams, albedo, nvox):
     import voxelrt_amd as V
     from voxelrt_amd import workloads as W
 
-    world, scene, cams, albedo, nvox = c5_small
     g = torch.Generator(device="cuda").manual_seed(6)
     target_albedo = (albedo + 0.05 * torch.randn(albedo.shape, generator=g, device="cuda")).contiguous()
     target, _, _ = V.render_saved(world, target_albedo, cams, W.MAX_ITERS)
@@ -116,6 +128,35 @@ def test_backward_matches_reference_and_finite_differences(vrt, oracle_mod, cuda
             assert abs(grad[v, c] - fd) <= tol, (v, c, grad[v, c], fd, abssum[v, c])
             checked += 1
     assert checked >= 3 * min(256, len(hit_ids))
+
+
+@pytest.fixture(scope="module")
+def c5_full():
+    """The differentiable configuration at full size: 512^3 terrain, 16 views of 512x512."""
+    import voxelrt_amd as V
+    from voxelrt_amd import workloads as W
+
+    scene = W.C5_WORLD.build()
+    world = V.World.from_scene(scene)
+    cams = W.c5_cameras(n=16, size=512)
+    nvox = world.info().num_bricks * 512
+    g = torch.Generator(device="cuda").manual_seed(5)
+    albedo = torch.rand((nvox, 3), generator=g, device="cuda", dtype=torch.float32)
+    return world, scene, cams, albedo, nvox
+
+
+def test_full_configuration_forward_matches_oracle(vrt, oracle_mod, cuda, c5_full):
+    import voxelrt_amd as V
+    from voxelrt_amd import workloads as W
+
+    world, scene, cams, albedo, nvox = c5_full
+    assert check_forward(V, W, oracle_mod, world, scene, cams, albedo) == 16 * 512 * 512
+
+
+def test_full_configuration_backward_matches_reference(vrt, oracle_mod, cuda, c5_full):
+    """16 x 512^2 pixels into 49.9 M albedo voxels: the gradient against the FP64 CPU reference
+    gradient and FP64 central differences on 256 voxels, as for the small views."""
+    check_backward(oracle_mod, *c5_full)
 
 
 def test_autograd_function(vrt, cuda, c5_small):

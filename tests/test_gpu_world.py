@@ -246,3 +246,173 @@ def test_maximum_extent_world(vrt, cuda):
             normal = tuple(((info >> (8 + 2 * a)) & 3 ^ 2) - 2 for a in range(3))
             assert (tuple(rec[k, :3]), info & 0xFF, normal) == ((X, Y, Z), 9, nk), (W, k)
             assert t[k] == np.float32(tk), (W, k, t[k], tk)
+
+
+def brick_alloc(vox: np.ndarray, si: int) -> tuple[int, list]:
+    """Allocation mask of sector si of a dense [Y][Z][X] grid and its 64 bricks' bytes."""
+    sy, sz, sx = vox.shape
+    nxz = sx // 32
+    sh = nxz.bit_length() - 1
+    cx, cz, cy = si & (nxz - 1), (si >> sh) & (nxz - 1), si >> (2 * sh)
+    sec = vox[cy * 32:(cy + 1) * 32, cz * 32:(cz + 1) * 32, cx * 32:(cx + 1) * 32]
+    m, data = 0, []
+    for b in range(64):
+        bx, bz, by = b & 3, (b >> 2) & 3, b >> 4
+        br = sec[by * 8:(by + 1) * 8, bz * 8:(bz + 1) * 8, bx * 8:(bx + 1) * 8].reshape(-1)
+        data.append(br)
+        if br.any():
+            m |= 1 << b
+    return m, data
+
+
+def dirty_stream(vox: np.ndarray, dirty: dict):
+    """(sector_idx, alloc_mask, dirty_mask, bricks) for {sector: dirty brick mask}: only the dirty
+    bricks that stay allocated are sent, in increasing brick index (dirtyMask &= allocMask)."""
+    idx, am, dm, bricks = [], [], [], []
+    for si, dmask in dirty.items():
+        m, data = brick_alloc(vox, si)
+        idx.append(si)
+        am.append(m)
+        dm.append(dmask)
+        bricks += [data[b] for b in range(64) if (dmask & m) >> b & 1]
+    br = np.concatenate(bricks) if bricks else np.zeros(0, np.uint8)
+    return np.array(idx, np.uint32), np.array(am, np.uint64), np.array(dm, np.uint64), br
+
+
+def brick_of(x: int, y: int, z: int, nxz_log2: int = 2) -> tuple[int, int]:
+    si = (x >> 5) | (z >> 5) << nxz_log2 | (y >> 5) << (2 * nxz_log2)
+    bi = ((x >> 3) & 3) | ((z >> 3) & 3) << 2 | ((y >> 3) & 3) << 4
+    return si, bi
+
+
+def test_brick_sync_uploads_only_dirty_bricks(vrt, oracle_mod, cuda):
+    """DirtyLocs at brick granularity (GpuRenderer.cpp:52-79, 114-129): single-brick edits of full
+    sectors upload 512 bytes each; allocation changes move the kept bricks on the device; after
+    every edit the casts equal the oracle's on the edited grid and a freshly uploaded world's."""
+    import voxelrt_amd as V
+    from test_oracle import dense_world
+
+    rng = np.random.default_rng(17)
+    vox = blob_world(128, seed=41, density=0.02)
+    vox[:40] = np.where(vox[:40] == 0, 7, vox[:40])  # solid floor: sectors of the bottom layer are full
+    vox[20:28, 8:16, 72:80] = 0                       # one empty brick inside a full sector
+    nsec = 64
+    world = V.World(7, 7)
+    world.sync(*sector_stream_of(vox, list(range(nsec))))
+    o, d = random_rays(20000, 128, seed=19)
+
+    def check(tag):
+        ref = dense_world(oracle_mod, vox).cast(o, d, (0, 0, 0), 128)
+        t, rec = cast_np(world, o, d)
+        assert_hits_equal(t, rec, ref["t"], ref["rec"], tag)
+        fresh = V.World.from_scene(V.Scene.from_dense(vox))
+        assert np.array_equal(rec, cast_np(fresh, o, d)[1]), tag
+        masks = sector_stream_of(vox, list(range(nsec)))[1]
+        check_sector_levels(world, dict(zip(range(nsec), masks)))
+        assert world.info().num_bricks == sum(bin(int(m)).count("1") for m in masks), tag
+
+    check("initial")
+    full_si, _ = brick_of(4, 4, 4)
+    assert brick_alloc(vox, full_si)[0] == (1 << 64) - 1
+
+    def sync(dirty, tag, bricks_up, moved):
+        nonlocal vox
+        i0 = world.info()
+        world.sync_bricks(*dirty_stream(vox, dirty))
+        i1 = world.info()
+        assert i1.upload_brick_bytes - i0.upload_brick_bytes == 512 * bricks_up, tag
+        assert (i1.moved_bricks - i0.moved_bricks > 0) == moved, tag
+        # the brick bytes and their slots, the records and the changed sector-of-sectors words
+        assert i1.upload_bytes - i0.upload_bytes <= 516 * bricks_up + 8 * (i1.moved_bricks - i0.moved_bricks) \
+            + 48 * len(dirty) + 64, tag
+        check(tag)
+
+    # 1. voxels repainted inside one brick of a full sector: that brick alone
+    vox[9, 10, 11] = 200
+    vox[12, 13, 14] = 0
+    sync({brick_of(11, 9, 10)[0]: 1 << brick_of(11, 9, 10)[1]}, "repaint", 1, False)
+    # 2. a voxel in the empty brick: the allocation grows in the middle, kept bricks shift
+    vox[22, 10, 75] = 5
+    si, bi = brick_of(75, 22, 10)
+    sync({si: 1 << bi}, "new brick", 1, True)
+    # 3. a brick cleared: it leaves the allocation, nothing uploaded, the bricks after it shift
+    vox[0:8, 0:8, 0:8] = 0
+    si, bi = brick_of(0, 0, 0)
+    sync({si: 1 << bi}, "cleared brick", 0, True)
+    # 4. several sectors in one sync, random single voxels, dirty masks listing some clean bricks
+    dirty = {}
+    for _ in range(12):
+        x, y, z = (int(c) for c in rng.integers(0, 128, 3))
+        vox[y, z, x] = int(rng.integers(0, 256))
+        s, b = brick_of(x, y, z)
+        dirty[s] = dirty.get(s, 0) | 1 << b
+    s0 = next(iter(dirty))
+    dirty[s0] |= 1 << 63  # a clean brick listed as dirty is sent again (if allocated)
+    n_up = sum(bin(d & brick_alloc(vox, s)[0]).count("1") for s, d in dirty.items())
+    i0 = world.info()
+    world.sync_bricks(*dirty_stream(vox, dirty))
+    assert world.info().upload_brick_bytes - i0.upload_brick_bytes == 512 * n_up
+    check("scattered edits")
+    # 5. a whole sector emptied
+    vox[32:64, 32:64, 32:64] = 0
+    si = 1 | 1 << 2 | 1 << 4
+    sync({si: (1 << 64) - 1}, "emptied sector", 0, False)
+    # a brick newly allocated but not dirty is rejected and leaves the world unchanged
+    vox2 = vox.copy()
+    vox2[40, 40, 40] = 3
+    si, bi = brick_of(40, 40, 40)
+    idx, am, dm, br = dirty_stream(vox2, {si: 0})
+    with pytest.raises(V.VrtError, match="not dirty"):
+        world.sync_bricks(idx, am, dm, br)
+    check("rejected")
+
+
+def test_sync_staging_failure_leaves_world_unchanged(vrt, cuda):
+    """The staging buffers are allocated by a world's first sync, before any device work: a failure
+    there returns an error and the world stays as it was (empty), usable by the next sync."""
+    import voxelrt_amd as V
+    from voxelrt_amd import _lib as L
+
+    vox = blob_world(64, seed=8, density=0.01)
+    o, d = random_rays(4000, 64, seed=2)
+    world = V.World(6, 6)
+    os.environ["VRT_FAULT_INJECT"] = "vrt_world_sync:staging"
+    try:
+        with pytest.raises(V.VrtError) as e:
+            world.sync(*sector_stream_of(vox, list(range(8))))
+        assert e.value.status == L.VRT_ERR_HIP
+    finally:
+        del os.environ["VRT_FAULT_INJECT"]
+    assert world.info().num_bricks == 0
+    t, rec = cast_np(world, o, d)
+    assert np.all((rec[:, 3] & 0xFF) == 0)
+    world.sync(*sector_stream_of(vox, list(range(8))))
+    fresh = V.World.from_scene(V.Scene.from_dense(vox))
+    assert np.array_equal(cast_np(world, o, d)[1], cast_np(fresh, o, d)[1])
+
+
+def test_sync_device_failure_marks_world_failed(vrt, cuda):
+    """A HIP failure once a sync's device work has begun leaves the device tables part-written:
+    the call reports it, and every later call on the world returns VRT_ERR_STATE."""
+    import voxelrt_amd as V
+    from voxelrt_amd import _lib as L
+
+    vox = blob_world(64, seed=8, density=0.01)
+    world = V.World(6, 6)
+    world.sync(*sector_stream_of(vox, list(range(8))))
+    o, d = random_rays(64, 64, seed=1)
+    rb = V.RayBatch.from_numpy(o, d)
+    vox[1, 2, 3] = 9
+    os.environ["VRT_FAULT_INJECT"] = "vrt_world_sync:device"
+    try:
+        with pytest.raises(V.VrtError) as e:
+            world.sync(*sector_stream_of(vox, [0]))
+        assert e.value.status == L.VRT_ERR_HIP and "unusable" in str(e.value)
+    finally:
+        del os.environ["VRT_FAULT_INJECT"]
+    torch.cuda.synchronize()
+    for call in (lambda: world.cast(rb, 128), lambda: world.sync(*sector_stream_of(vox, [0])),
+                 lambda: world.set_palette(np.zeros(256, np.uint64))):
+        with pytest.raises(V.VrtError) as e:
+            call()
+        assert e.value.status == L.VRT_ERR_STATE and "destroy" in str(e.value)

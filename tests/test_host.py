@@ -30,7 +30,7 @@ def test_library_exports_every_declared_symbol(vrt):
 
 def test_abi_version_and_error_string(vrt):
     lib = vrt.lib()
-    assert lib.vrt_abi_version() == 1
+    assert lib.vrt_abi_version() == 2
     assert isinstance(lib.vrt_last_error(), bytes)
 
 
@@ -47,6 +47,24 @@ def test_null_arguments_are_rejected(vrt):
     out = C.c_void_p()
     assert lib.vrt_world_create(C.byref(desc), None, C.byref(out)) == L.VRT_ERR_INVALID_ARGUMENT
     assert b"5..14" in lib.vrt_last_error()
+    assert lib.vrt_world_sync_bricks(None, 0, None, None, None, None, None) == L.VRT_ERR_INVALID_ARGUMENT
+
+
+def test_bounce_generation_rejects_a_wrapping_primary_count(vrt):
+    """n above 2^32 - 4096 would wrap the 32-bit block counts: rejected before any launch (the
+    pointers are placeholders that are never dereferenced)."""
+    from voxelrt_amd import _lib as L
+
+    lib = vrt.lib()
+    fake = 0x1000
+    prim = L.Rays(fake, fake, fake, fake, fake, fake, (L.i32 * 3)(0, 0, 0), 0xFFFFFFFF)
+    hits = L.Hits(fake, fake, fake)
+    out = L.RayBuffers(fake, fake, fake, fake, fake, fake)
+    rc = lib.vrt_bounce_generate(C.byref(prim), C.byref(hits), 1920, 1, C.byref(out), fake, fake, fake, None)
+    assert rc == L.VRT_ERR_INVALID_ARGUMENT
+    assert b"2^32 - 4096" in lib.vrt_last_error()
+    # the scratch size is computed in 64 bits: one u32 per 256 primaries, plus one
+    assert lib.vrt_bounce_scratch_bytes(0xFFFFFFFF) == ((0xFFFFFFFF + 255) // 256 + 1) * 4
 
 
 def test_terrain_generator_is_deterministic(vrt):

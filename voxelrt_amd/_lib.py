@@ -20,9 +20,6 @@ VRT_ERR_STATE = 5
 VRT_SCHED_AUTO = 0
 VRT_SCHED_GRID = 1
 VRT_SCHED_PERSISTENT = 2
-VRT_SCHED_MULTIPASS = 3
-VRT_SCHED_REPACK = 4
-VRT_SCHED_LANES = 5
 
 VRT_ORDER_FORWARD = 0
 VRT_ORDER_REVERSE = 1
@@ -63,7 +60,8 @@ class WorldDesc(C.Structure):
 class WorldInfo(C.Structure):
     _fields_ = [("size_xz", u32), ("size_y", u32), ("sectors_xz", u32), ("sectors_y", u32),
                 ("num_sectors", u32), ("num_bricks", u32), ("slot_capacity", u32), ("device_bytes", u64),
-                ("slot_high_water", u32)]
+                ("slot_high_water", u32), ("generation", u32), ("upload_bytes", u64),
+                ("upload_brick_bytes", u64), ("moved_bricks", u64)]
 
 
 class WorldBuffers(C.Structure):
@@ -89,8 +87,7 @@ class CastCounters(C.Structure):
 
 class CastOpts(C.Structure):
     _fields_ = [("schedule", u32), ("refill", u32), ("chunk", u32), ("waves_per_cu", u32),
-                ("counters", P), ("count", P), ("pass_iters", u32), ("passes", u32), ("scratch", P),
-                ("scratch_bytes", C.c_size_t), ("order", u32)]
+                ("counters", P), ("count", P), ("order", u32), ("reserved", u32)]
 
 
 class HitF64(C.Structure):
@@ -168,11 +165,11 @@ _SIGNATURES = {
     "vrt_world_destroy": (C.c_int, [P]),
     "vrt_world_get_info": (C.c_int, [P, C.POINTER(WorldInfo)]),
     "vrt_world_sync": (C.c_int, [P, u32, P, P, P, P]),
+    "vrt_world_sync_bricks": (C.c_int, [P, u32, P, P, P, P, P]),
     "vrt_world_set_palette": (C.c_int, [P, P, P]),
     "vrt_world_get_buffers": (C.c_int, [P, C.POINTER(WorldBuffers)]),
     "vrt_cast": (C.c_int, [P, C.POINTER(Rays), u32, C.POINTER(Hits), P]),
     "vrt_cast_ex": (C.c_int, [P, C.POINTER(Rays), u32, C.POINTER(Hits), C.POINTER(CastOpts), P]),
-    "vrt_cast_scratch_bytes": (C.c_size_t, [u32]),
     "vrt_cast_one_f64": (C.c_int, [P, C.POINTER(C.c_double * 3), C.POINTER(C.c_double * 3), u32, C.POINTER(HitF64)]),
     "vrt_bounce_scratch_bytes": (C.c_size_t, [u32]),
     "vrt_bounce_generate": (C.c_int, [C.POINTER(Rays), C.POINTER(Hits), u32, u32, C.POINTER(RayBuffers), P, P, P, P]),

@@ -303,6 +303,17 @@ class World:
         L.check(L.lib().vrt_world_sync(self._h, len(si), si.ctypes.data, am.ctypes.data,
                                        br.ctypes.data if br.size else 0, _stream_handle(stream)))
 
+    def sync_bricks(self, sector_idx: np.ndarray, alloc_mask: np.ndarray, dirty_mask: np.ndarray,
+                    bricks: np.ndarray, stream=None) -> None:
+        """DirtyLocs sync (GpuRenderer.cpp:45-167): bricks holds only the dirty bricks of each
+        sector (dirty_mask & alloc_mask), sector after sector, in increasing brick index."""
+        si = np.ascontiguousarray(sector_idx, dtype=np.uint32)
+        am = np.ascontiguousarray(alloc_mask, dtype=np.uint64)
+        dm = np.ascontiguousarray(dirty_mask, dtype=np.uint64)
+        br = np.ascontiguousarray(bricks, dtype=np.uint8)
+        L.check(L.lib().vrt_world_sync_bricks(self._h, len(si), si.ctypes.data, am.ctypes.data, dm.ctypes.data,
+                                              br.ctypes.data if br.size else 0, _stream_handle(stream)))
+
     def set_palette(self, palette: np.ndarray, stream=None) -> None:
         p = np.ascontiguousarray(palette, dtype=np.uint64)
         assert p.shape == (256,)
@@ -321,7 +332,6 @@ class World:
     def cast(self, rays: RayBatch, max_iters: int = 128, with_pos: bool = False, schedule: int = 0,
              chunk: int = 0, waves_per_cu: int = 0, counters: torch.Tensor | None = None,
              out: HitBatch | None = None, count: torch.Tensor | None = None, refill: int = 0,
-             pass_iters: int = 0, passes: int = 0, scratch: torch.Tensor | None = None,
              order: int = 0, stream=None) -> HitBatch:
         """Trace rays (RayCast, CpuRenderer.cpp:172-224).  count: optional device int32 holding the
         number of rays to trace (rays.n is then the capacity); order: the grid's block order
@@ -330,9 +340,7 @@ class World:
         r = rays.to_c()
         h = hits.to_c()
         opts = L.CastOpts(schedule, refill, chunk, waves_per_cu, counters.data_ptr() if counters is not None else 0,
-                          count.data_ptr() if count is not None else 0, pass_iters, passes,
-                          scratch.data_ptr() if scratch is not None else 0,
-                          scratch.numel() * scratch.element_size() if scratch is not None else 0, order)
+                          count.data_ptr() if count is not None else 0, order, 0)
         L.check(L.lib().vrt_cast_ex(self._h, C.byref(r), max_iters, C.byref(h), C.byref(opts),
                                     _stream_handle(stream)))
         return hits

@@ -238,6 +238,7 @@ struct vrt_batch_plan {
     hipStream_t capture_stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
+    uint32_t graph_generation = 0;  // the world's generation when the graph was captured
 };
 
 static void destroy_plan(vrt_batch_plan* p) {
@@ -295,6 +296,7 @@ static int launch_plan(vrt_batch_plan* p, vrt_cast_counters* ctr, hipStream_t s)
 static int create_plan(const vrt_world* world, const vrt_batch* b, const vrt_batch_tuning* tun, hipStream_t s,
                        vrt_batch_plan** out) {
     if (!world || !b || !tun || !out) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_batch_plan_create: null argument");
+    if (int rcw = check_world(world, "vrt_batch_plan_create"); rcw != VRT_OK) return rcw;
     *out = nullptr;
     int rc = validate_tuning(*tun);
     if (rc != VRT_OK) return rc;
@@ -325,6 +327,9 @@ static int create_plan(const vrt_world* world, const vrt_batch* b, const vrt_bat
                 return fail(VRT_ERR_INVALID_ARGUMENT, "batch: camera size must be a multiple of its tile");
             if (c.width % tun->tile_w || c.height % tun->tile_h)
                 return fail(VRT_ERR_INVALID_ARGUMENT, "batch: camera size must be a multiple of the tuned tile");
+            // pixel indices are 32-bit in the kernel
+            if ((uint64_t)c.width * c.height > 0xFFFFFF00ull)
+                return fail(VRT_ERR_INVALID_ARGUMENT, "batch: camera with more than 2^32 - 256 pixels");
             d.count = c.width * c.height;
             if (g.count_ptr) return fail(VRT_ERR_INVALID_ARGUMENT, "batch: camera segments have a fixed count");
         } else if (g.kind == VRT_SEG_RAYS) {
@@ -333,8 +338,17 @@ static int create_plan(const vrt_world* world, const vrt_batch* b, const vrt_bat
         } else {
             return fail(VRT_ERR_INVALID_ARGUMENT, "batch: unknown segment kind");
         }
+        // hit_offset + index and ray_offset + index are 32-bit in the kernel
+        if ((uint64_t)d.hit_offset + d.count > 0x100000000ull)
+            return fail(VRT_ERR_INVALID_ARGUMENT, "batch: segment " + std::to_string(i) +
+                                                      " reaches past hit index 2^32 - 1");
+        if (g.kind == VRT_SEG_RAYS && (uint64_t)d.ray_offset + d.count > 0x100000000ull)
+            return fail(VRT_ERR_INVALID_ARGUMENT, "batch: segment " + std::to_string(i) +
+                                                      " reaches past ray index 2^32 - 1");
+        if (d.count > 0xFFFFFF00u)
+            return fail(VRT_ERR_INVALID_ARGUMENT, "batch: segment " + std::to_string(i) + " has more than 2^32 - 256 rays");
         d.block_begin = (uint32_t)blocks;
-        blocks += (d.count + per_block - 1) / per_block;
+        blocks += ((uint64_t)d.count + per_block - 1) / per_block;
         rays += d.count;
         if (blocks > 0x7FFFFFFFull) return fail(VRT_ERR_INVALID_ARGUMENT, "batch: too many rays");
     }
@@ -345,7 +359,7 @@ static int create_plan(const vrt_world* world, const vrt_batch* b, const vrt_bat
     p->tuning = *tun;
     p->total_rays = rays;
     int cus = 256;
-    rc = device_info(world->device, &cus, nullptr);
+    rc = device_info(world->device, &cus, nullptr, s);
     if (rc != VRT_OK) {
         destroy_plan(p);
         return rc;
@@ -443,6 +457,7 @@ int vrt_batch_plan_create(const vrt_world* world, const vrt_batch* batch, const 
 int vrt_batch_launch(vrt_batch_plan* plan, vrt_cast_counters* counters, void* stream) {
     return guarded([&]() -> int {
         if (!plan) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_batch_launch: null plan");
+        if (int rcw = check_world(plan->world, "vrt_batch_launch"); rcw != VRT_OK) return rcw;
         if (fault_injected("vrt_batch_launch")) return fail(VRT_ERR_HIP, "vrt_batch_launch: injected HIP fault");
         return launch_plan(plan, counters, (hipStream_t)stream);
     });
@@ -451,10 +466,21 @@ int vrt_batch_launch(vrt_batch_plan* plan, vrt_cast_counters* counters, void* st
 int vrt_batch_graph_launch(vrt_batch_plan* plan, void* stream) {
     return guarded([&]() -> int {
         if (!plan) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_batch_graph_launch: null plan");
+        if (int rcw = check_world(plan->world, "vrt_batch_graph_launch"); rcw != VRT_OK) return rcw;
         if (plan->args.nblocks == 0) return VRT_OK;  // nothing to cast, nothing to capture
+        if (plan->graph_exec && plan->graph_generation != plan->world->generation) {
+            // the world's occupancy and voxel buffers moved (slot growth) since the capture: the
+            // graph's kernel arguments point at freed memory, so capture again
+            VRT_HIP(hipDeviceSynchronize());  // earlier replays may be in flight on any stream
+            (void)hipGraphExecDestroy(plan->graph_exec);
+            (void)hipGraphDestroy(plan->graph);
+            plan->graph_exec = nullptr;
+            plan->graph = nullptr;
+        }
         if (!plan->graph_exec) {
             // capture on a private stream (the null stream cannot be captured)
-            VRT_HIP(hipStreamCreateWithFlags(&plan->capture_stream, hipStreamNonBlocking));
+            if (!plan->capture_stream) VRT_HIP(hipStreamCreateWithFlags(&plan->capture_stream, hipStreamNonBlocking));
+            plan->graph_generation = plan->world->generation;
             VRT_HIP(hipStreamBeginCapture(plan->capture_stream, hipStreamCaptureModeThreadLocal));
             int rc = launch_plan(plan, nullptr, plan->capture_stream);
             hipGraph_t g = nullptr;
@@ -485,6 +511,7 @@ int vrt_batch_autotune(const vrt_world* world, const vrt_batch* batch, const cha
                        vrt_batch_tuning* best, float* best_ms, void* stream) {
     return guarded([&]() -> int {
         if (!world || !batch || !best) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_batch_autotune: null argument");
+        if (int rcw = check_world(world, "vrt_batch_autotune"); rcw != VRT_OK) return rcw;
         hipStream_t s = (hipStream_t)stream;
         std::string key = device_name(world->device) + ":" + batch_signature(world, batch);
         {

@@ -153,6 +153,7 @@ int vrt_render_fwd(const vrt_world* world, const vrt_camera* cameras, uint32_t n
     return guarded([&]() -> int {
         if (!world || (nviews && !cameras) || !light_dir)
             return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_render_fwd: null argument");
+        if (int rcw = check_world(world, "vrt_render_fwd"); rcw != VRT_OK) return rcw;
         if (max_iters == 0 || max_iters > 65535) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_render_fwd: max_iters must be 1..65535");
         if ((uint64_t)world->slot_cap * 512 >= 0xFFFFFFFFull)
             return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_render_fwd: world too large for 32-bit voxel ids");
@@ -198,6 +199,7 @@ int vrt_render_bwd(const vrt_world* world, uint32_t npixels, const float* grad_i
     return guarded([&]() -> int {
         if (!world || (npixels && (!grad_image || !saved_voxel || !saved_weight || !grad_albedo)))
             return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_render_bwd: null argument");
+        if (int rcw = check_world(world, "vrt_render_bwd"); rcw != VRT_OK) return rcw;
         if (npixels == 0) return VRT_OK;
         if (npixels > 0xFFFFFF00u) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_render_bwd: more than 2^32 - 256 pixels");
         render_bwd_kernel<<<(npixels + 255) / 256, 256, 0, (hipStream_t)stream>>>(npixels, grad_image, saved_voxel,

@@ -263,6 +263,7 @@ extern "C" int vrt_cast_frame(const vrt_world* world, const vrt_camera* cam, uin
                               vrt_hits* primary, vrt_hits* bounce, const vrt_frame_opts* opts, void* stream) {
     return guarded([&]() -> int {
         if (!world || !cam || !primary || !bounce) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_cast_frame: null argument");
+        if (int rcw = check_world(world, "vrt_cast_frame"); rcw != VRT_OK) return rcw;
         if (max_iters == 0 || max_iters > 65535) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_cast_frame: max_iters must be 1..65535");
         if (!cam->width || !cam->height || !cam->tile_w || !cam->tile_h || cam->width % cam->tile_w ||
             cam->height % cam->tile_h)
@@ -312,7 +313,7 @@ extern "C" int vrt_cast_frame(const vrt_world* world, const vrt_camera* cam, uin
         if (chunk % 64 != 0 || chunk > (1u << 20)) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_cast_frame: chunk must be a multiple of 64");
         int cus = 0;
         uint32_t* queue = nullptr;
-        int rc = device_info(world->device, &cus, &queue);
+        int rc = device_info(world->device, &cus, &queue, s);
         if (rc != VRT_OK) return rc;
         uint32_t blocks = std::max<uint32_t>(kQueues, ((uint32_t)cus * 32 / 4) / kQueues * kQueues);
         ChunkQueue cq;

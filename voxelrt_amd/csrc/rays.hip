@@ -142,6 +142,9 @@ __global__ __launch_bounds__(256) void bounce_write_kernel(const int4* __restric
 static const uint32_t kGenRows = 4;
 static const uint32_t kGenBlock = 1024 * kGenRows;
 static const uint32_t kGenMaxBlocks = 2048;
+// Largest primary count: the block counts (n + kGenBlock - 1) / kGenBlock and
+// (n + kBounceBlock - 1) / kBounceBlock are computed in 32 bits.
+static const uint32_t kMaxBouncePrimaries = 0xFFFFF000u;
 
 __global__ __launch_bounds__(1024) void bounce_count_rows_kernel(const int4* __restrict__ rec, uint32_t n,
                                                                  uint32_t* __restrict__ block_counts) {
@@ -257,7 +260,7 @@ int vrt_camera_rays(const vrt_camera* cam, vrt_ray_buffers* out, void* stream) {
 }
 
 size_t vrt_bounce_scratch_bytes(uint32_t n) {
-    return (size_t)((n + kBounceBlock - 1) / kBounceBlock + 1) * sizeof(uint32_t);
+    return (size_t)(((uint64_t)n + kBounceBlock - 1) / kBounceBlock + 1) * sizeof(uint32_t);
 }
 
 int vrt_bounce_generate(const vrt_rays* primary, const vrt_hits* hits, uint32_t width, uint32_t seed,
@@ -268,6 +271,9 @@ int vrt_bounce_generate(const vrt_rays* primary, const vrt_hits* hits, uint32_t 
         if (width == 0) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_bounce_generate: width must be > 0");
         hipStream_t s = (hipStream_t)stream;
         uint32_t n = primary->n;
+        // block counts below are 32-bit: (n + 4095) / 4096 and (n + 255) / 256 must not wrap
+        if (n > kMaxBouncePrimaries)
+            return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_bounce_generate: at most 2^32 - 4096 primary rays");
         if (n && (!src || !scratch || !hits->rec)) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_bounce_generate: null argument");
         if (n && !hits->pos) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_bounce_generate: hits need positions");
         if (n && (!out->ox || !out->oy || !out->oz || !out->dx || !out->dy || !out->dz))

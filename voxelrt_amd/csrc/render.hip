@@ -158,6 +158,7 @@ extern "C" int vrt_render_frame(const vrt_world* world, const vrt_camera* cam, c
                                 vrt_gbuffer* out, void* stream) {
     return guarded([&]() -> int {
         if (!world || !cam || !opts || !out) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_render_frame: null argument");
+        if (int rcw = check_world(world, "vrt_render_frame"); rcw != VRT_OK) return rcw;
         if (!cam->width || !cam->height) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_render_frame: empty image");
         if ((uint64_t)cam->width * cam->height > 0x7FFFFFFFull)
             return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_render_frame: too many pixels");

@@ -39,6 +39,7 @@ extern "C" int vrt_cast_shader(const vrt_world* world, const vrt_rays* rays, con
                                vrt_hits* hits, void* stream) {
     return guarded([&]() -> int {
         if (!world || !opts) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_cast_shader: null world or options");
+        if (int rcw = check_world(world, "vrt_cast_shader"); rcw != VRT_OK) return rcw;
         if (opts->method > VRT_SHADER_COARSE) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_cast_shader: unknown method");
         if (opts->ray_cell_lut > 1 || opts->reserved != 0)
             return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_cast_shader: ray_cell_lut must be 0 or 1, reserved 0");

@@ -68,7 +68,8 @@ int check_rays_hits(const vrt_rays* rays, const vrt_hits* hits, RaysIn& r, HitsO
 // Launches a cast with the given options on stream s.
 int launch_cast(const vrt_world* world, const RaysIn& r, const HitsOut& h, uint32_t max_iters,
                 const vrt_cast_opts& o, hipStream_t s);
-// Compute units of device `dev` and, when queue is not null, a queue counter from the device's ring.
-int device_info(int dev, int* cus, uint32_t** queue);
+// Compute units of device `dev` and, when queue is not null, the queue counters of persistent
+// launches on stream s of that device (kQueues counters, kQueueStride apart).
+int device_info(int dev, int* cus, uint32_t** queue, hipStream_t s);
 
 }  // namespace vrt

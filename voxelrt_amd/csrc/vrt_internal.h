@@ -128,6 +128,27 @@ struct vrt_world {
     std::vector<uint64_t> h_groups;
     uint64_t* d_groups = nullptr;
     uint64_t* d_lut = nullptr;
+    // bumped whenever d_occ / d_vox move (slot growth): device pointers taken earlier, e.g. in a
+    // captured graph, are stale once it changes
+    uint32_t generation = 0;
+    // set when a sync fails after its device work began: the device tables may then disagree
+    // with the host mirror, and every later call on the world returns VRT_ERR_STATE
+    bool failed = false;
+    // sync staging, owned by the world and reused by every sync: two pinned host halves and two
+    // device halves of kSyncHalf bytes (a chunk is packed into one half while the other's copy
+    // and apply kernel run); stage_ev[h] is recorded after the last kernel that reads half h
+    uint8_t* h_stage = nullptr;
+    uint8_t* d_stage = nullptr;
+    hipEvent_t stage_ev[2] = {nullptr, nullptr};
+    bool stage_pending[2] = {false, false};
+    uint32_t stage_next = 0;
+    // bricks whose slot changes while their contents stay: gathered here, then written back
+    uint8_t* d_moves = nullptr;
+    size_t moves_cap = 0;
+    hipEvent_t moves_ev = nullptr;
+    bool moves_pending = false;
+    // totals since creation
+    uint64_t upload_bytes = 0, upload_brick_bytes = 0, moved_bricks = 0;
 
     vrt::WorldView view() const {
         vrt::WorldView v;
@@ -152,6 +173,9 @@ struct vrt_world {
 };
 
 namespace vrt {
-// Launches the occupancy rebuild of `nslots` slots listed in d_slots (device) on stream.
-hipError_t launch_build_occupancy(vrt_world* w, const uint32_t* d_slots, uint32_t nslots, hipStream_t s);
+// Bytes of one sync staging half (host pinned and device).
+constexpr size_t kSyncHalf = (size_t)8 << 20;
+
+// VRT_OK, or VRT_ERR_STATE (with the message) when a failed sync left the world unusable.
+int check_world(const vrt_world* w, const char* entry);
 }  // namespace vrt

@@ -1,4 +1,4 @@
-// world.hip - device world lifecycle, slot arena, sector sync and occupancy build.
+// world.hip - device world lifecycle, slot arena, brick sync and occupancy build.
 //
 // Replaces GpuVoxelStorage (src/VoxelRT/GpuRenderer.cpp:17-167), its slot
 // allocator (src/VoxelRT/BrickSlotAllocator.{h,cpp}) and the occupancy build
@@ -108,20 +108,12 @@ uint32_t SlotArena::high_water() const {
     return capacity_;
 }
 
-// ---------------------------------------------------------------- occupancy build
+// ---------------------------------------------------------------- sync kernels
 
-// One thread per (brick, 4x4x4 cell): bit vx + 4 vz + 16 vy of cell
-// cx | cz << 1 | cy << 2, as in CpuRenderer.cpp:63-83 (UpdateOccupancy.comp
-// writes the same 64-bit mask as two 4x2x4 halves).
-__global__ __launch_bounds__(256) void build_occupancy_kernel(const uint8_t* __restrict__ vox,
-                                                              uint64_t* __restrict__ occ,
-                                                              const uint32_t* __restrict__ slots,
-                                                              uint32_t nslots) {
-    uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t b = gid >> 3, cell = gid & 7;
-    if (b >= nslots) return;
-    uint32_t slot = slots[b];
-    const uint8_t* brick = vox + (size_t)slot * 512;
+// Occupancy mask of cell `cell` (cx | cz << 1 | cy << 2) of a 512-byte brick: bit vx + 4 vz + 16 vy,
+// as in CpuRenderer.cpp:63-83 (UpdateOccupancy.comp:9-35 writes the same 64-bit mask as two
+// 4x2x4 halves).
+__device__ __forceinline__ uint64_t cell_occupancy(const uint8_t* brick, uint32_t cell) {
     uint32_t cx = (cell & 1) * 4, cz = ((cell >> 1) & 1) * 4, cy = (cell >> 2) * 4;
     uint64_t m = 0;
 #pragma unroll
@@ -133,22 +125,68 @@ __global__ __launch_bounds__(256) void build_occupancy_kernel(const uint8_t* __r
             for (uint32_t vx = 0; vx < 4; vx++)
                 if ((row >> (8 * vx)) & 0xFFu) m |= 1ull << (vx + 4 * vz + 16 * vy);
         }
-    occ[(size_t)slot * 8 + cell] = m;
+    return m;
 }
 
-hipError_t launch_build_occupancy(vrt_world* w, const uint32_t* d_slots, uint32_t nslots, hipStream_t s) {
-    if (nslots == 0) return hipSuccess;
-    uint64_t threads = (uint64_t)nslots * 8;
-    uint32_t blocks = (uint32_t)((threads + 255) / 256);
-    build_occupancy_kernel<<<blocks, 256, 0, s>>>(w->d_vox, w->d_occ, d_slots, nslots);
-    return hipGetLastError();
+// Uploaded bricks: thread (k, part), 8 per brick, copies bytes [64 part, 64 part + 64) of staged
+// brick k to its slot and builds the occupancy mask of cell `part` from the staged bytes.
+__global__ __launch_bounds__(256) void sync_upload_kernel(uint8_t* __restrict__ vox, uint64_t* __restrict__ occ,
+                                                          const uint8_t* __restrict__ bricks,
+                                                          const uint32_t* __restrict__ slots, uint32_t n) {
+    uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t k = gid >> 3, part = gid & 7;
+    if (k >= n) return;
+    uint32_t slot = slots[k];
+    const uint8_t* src = bricks + (size_t)k * 512;
+    const uint4* s4 = (const uint4*)(src + part * 64);
+    uint4* d4 = (uint4*)(vox + (size_t)slot * 512 + part * 64);
+#pragma unroll
+    for (int j = 0; j < 4; j++) d4[j] = s4[j];
+    occ[(size_t)slot * 8 + part] = cell_occupancy(src, part);
 }
 
-// The records of the synced sectors (indices unique) into the sector table.
-__global__ __launch_bounds__(256) void scatter_records_kernel(uint4* __restrict__ sec, const uint4* __restrict__ rec,
-                                                              const uint32_t* __restrict__ idx, uint32_t n) {
+// Bricks that keep their contents but change slot (their sector's allocation changed), in two
+// launches so that no move reads a slot another move has written: thread (k, part), 9 per
+// brick, copies 64 bytes of slot src[k] (parts 0-7: voxels, part 8: the 8 occupancy masks)
+// into the move area, then back out into slot dst[k].
+__global__ __launch_bounds__(256) void sync_gather_kernel(const uint8_t* __restrict__ vox,
+                                                          const uint64_t* __restrict__ occ,
+                                                          const uint32_t* __restrict__ src, uint32_t n,
+                                                          uint8_t* __restrict__ area) {
+    uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t k = gid / 9, part = gid % 9;
+    if (k >= n) return;
+    size_t slot = src[k];
+    const uint4* s4 = part < 8 ? (const uint4*)(vox + slot * 512 + part * 64) : (const uint4*)(occ + slot * 8);
+    uint4* d4 = (uint4*)(area + (size_t)k * 576 + part * 64);
+#pragma unroll
+    for (int j = 0; j < 4; j++) d4[j] = s4[j];
+}
+
+__global__ __launch_bounds__(256) void sync_place_kernel(uint8_t* __restrict__ vox, uint64_t* __restrict__ occ,
+                                                         const uint32_t* __restrict__ dst, uint32_t n,
+                                                         const uint8_t* __restrict__ area) {
+    uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t k = gid / 9, part = gid % 9;
+    if (k >= n) return;
+    size_t slot = dst[k];
+    const uint4* s4 = (const uint4*)(area + (size_t)k * 576 + part * 64);
+    uint4* d4 = part < 8 ? (uint4*)(vox + slot * 512 + part * 64) : (uint4*)(occ + slot * 8);
+#pragma unroll
+    for (int j = 0; j < 4; j++) d4[j] = s4[j];
+}
+
+// Records of the synced sectors (indices unique) and the sector-of-sectors words they changed.
+__global__ __launch_bounds__(256) void sync_records_kernel(uint4* __restrict__ sec, const uint4* __restrict__ rec,
+                                                           const uint32_t* __restrict__ rec_idx, uint32_t nrec,
+                                                           uint64_t* __restrict__ groups,
+                                                           const uint64_t* __restrict__ gval,
+                                                           const uint32_t* __restrict__ gidx, uint32_t ngrp) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) sec[idx[i]] = rec[i];
+    if (i < nrec)
+        sec[rec_idx[i]] = rec[i];
+    else if (i - nrec < ngrp)
+        groups[gidx[i - nrec]] = gval[i - nrec];
 }
 
 }  // namespace vrt
@@ -184,6 +222,7 @@ static int grow_slots(vrt_world* w, uint32_t need, hipStream_t s) {
     w->d_occ = occ;
     w->d_vox = vox;
     w->slot_cap = cap;
+    w->generation++;
     return VRT_OK;
 }
 
@@ -194,8 +233,106 @@ static void free_world(vrt_world* w) {
     (void)hipFree(w->d_palette);
     (void)hipFree(w->d_groups);
     (void)hipFree(w->d_lut);
+    (void)hipFree(w->d_stage);
+    (void)hipFree(w->d_moves);
+    if (w->h_stage) (void)hipHostFree(w->h_stage);
+    for (hipEvent_t e : w->stage_ev)
+        if (e) (void)hipEventDestroy(e);
+    if (w->moves_ev) (void)hipEventDestroy(w->moves_ev);
     delete w;
 }
+
+int vrt::check_world(const vrt_world* w, const char* entry) {
+    if (w && w->failed)
+        return fail(VRT_ERR_STATE, std::string(entry) +
+                                       ": a vrt_world_sync failed after its device work began, so the world's "
+                                       "device tables are inconsistent; destroy the world");
+    return VRT_OK;
+}
+
+// ---------------------------------------------------------------- sync staging
+
+// The staging halves and their events, allocated by the first sync and kept.
+static int ensure_staging(vrt_world* w) {
+    if (w->h_stage) return VRT_OK;
+    if (fault_injected("vrt_world_sync:staging")) return fail(VRT_ERR_HIP, "vrt_world_sync: injected staging failure");
+    uint8_t* h = nullptr;
+    uint8_t* d = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    hipError_t e = hipHostMalloc((void**)&h, 2 * kSyncHalf, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMalloc(&d, 2 * kSyncHalf);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ev[0], hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ev[1], hipEventDisableTiming);
+    if (e != hipSuccess) {
+        if (h) (void)hipHostFree(h);
+        (void)hipFree(d);
+        for (hipEvent_t x : ev)
+            if (x) (void)hipEventDestroy(x);
+        return hip_fail(e, "vrt_world_sync: staging buffers");
+    }
+    w->h_stage = h;
+    w->d_stage = d;
+    w->stage_ev[0] = ev[0];
+    w->stage_ev[1] = ev[1];
+    return VRT_OK;
+}
+
+// The move area for n moved bricks: the source and destination slot lists (256-byte
+// aligned), then 576 bytes per brick.
+static size_t moves_bytes(uint32_t n) { return (((size_t)n * 8 + 255) & ~(size_t)255) + (size_t)n * 576; }
+
+static int ensure_moves(vrt_world* w, uint32_t n) {
+    if (n == 0) return VRT_OK;
+    if (!w->moves_ev) VRT_HIP(hipEventCreateWithFlags(&w->moves_ev, hipEventDisableTiming));
+    size_t need = moves_bytes(n);
+    if (w->moves_cap >= need) return VRT_OK;
+    if (w->moves_pending) VRT_HIP(hipEventSynchronize(w->moves_ev));  // the old area's last reader
+    w->moves_pending = false;
+    (void)hipFree(w->d_moves);
+    w->d_moves = nullptr;
+    w->moves_cap = 0;
+    size_t cap = std::bit_ceil(std::max<size_t>(need, (size_t)1 << 20));
+    VRT_HIP(hipMalloc(&w->d_moves, cap));
+    w->moves_cap = cap;
+    return VRT_OK;
+}
+
+// The next staging half, once the host may write its pinned part and the stream its device
+// part (the last kernel or copy that read it has finished).
+static hipError_t acquire_half(vrt_world* w, uint32_t& h) {
+    h = w->stage_next;
+    w->stage_next ^= 1u;
+    if (w->stage_pending[h]) {
+        hipError_t e = hipEventSynchronize(w->stage_ev[h]);
+        if (e != hipSuccess) return e;
+        w->stage_pending[h] = false;
+    }
+    return hipSuccess;
+}
+
+static hipError_t release_half(vrt_world* w, uint32_t h, hipStream_t s) {
+    hipError_t e = hipEventRecord(w->stage_ev[h], s);
+    if (e == hipSuccess) w->stage_pending[h] = true;
+    return e;
+}
+
+// Host bytes to device memory through the pinned halves, kSyncHalf at a time.
+static hipError_t stream_to_device(vrt_world* w, hipStream_t s, uint8_t* dst, const uint8_t* src, size_t bytes) {
+    for (size_t off = 0; off < bytes; off += kSyncHalf) {
+        size_t len = std::min(kSyncHalf, bytes - off);
+        uint32_t h;
+        hipError_t e = acquire_half(w, h);
+        if (e != hipSuccess) return e;
+        std::memcpy(w->h_stage + h * kSyncHalf, src + off, len);
+        e = hipMemcpyAsync(dst + off, w->h_stage + h * kSyncHalf, len, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = release_half(w, h, s);
+        if (e != hipSuccess) return e;
+        w->upload_bytes += len;
+    }
+    return hipSuccess;
+}
+
+static size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
 
 void vrt::ray_cell_lut(uint64_t table[512]) {
     for (uint32_t oct = 0; oct < 8; oct++)
@@ -211,17 +348,205 @@ void vrt::ray_cell_lut(uint64_t table[512]) {
         }
 }
 
-// Sector-of-sectors bit of sector idx, from its current mask.
-static void update_group_bit(vrt_world* w, uint32_t idx) {
-    uint32_t nxz = 1u << w->shift_xz;
-    uint32_t sx = idx & (nxz - 1u), sz = (idx >> w->shift_xz) & (nxz - 1u), sy = idx >> (2 * w->shift_xz);
-    uint32_t gm_xz = (1u << w->gshift_xz) - 1u, gm_y = (1u << w->gshift_y) - 1u;
-    uint32_t g = ((sx >> 2) & gm_xz) | ((sz >> 2) & gm_xz) << w->gshift_xz | ((sy >> 2) & gm_y) << (2 * w->gshift_xz);
-    uint64_t bit = 1ull << ((sx & 3u) | (sz & 3u) << 2 | (sy & 3u) << 4);
-    if (w->h_mask[idx])
-        w->h_groups[g] |= bit;
-    else
-        w->h_groups[g] &= ~bit;
+// ---------------------------------------------------------------- sync
+
+// The DirtyLocs protocol of GpuRenderer.cpp:45-167 for a set of sectors: sector sector_idx[i]
+// holds the bricks of alloc_mask[i] afterwards, and the bytes of those in dirty_mask[i] (all of
+// them when dirty_mask is NULL) follow in `bricks`, sector after sector, in increasing brick
+// index.  Bricks that stay allocated and are not dirty keep their device contents; when their
+// sector's slot run moves or their popcount position shifts (the reference then re-uploads the
+// whole sector, BrickSlotAllocator.cpp:7-23), they are moved on the device.  Every allocation
+// is made before any device work, and the host mirror is committed only after all device work
+// has been issued: a failure before the device work leaves the world unchanged, a failure
+// during it marks the world failed.
+static int world_sync(vrt_world* w, uint32_t nsec, const uint32_t* sector_idx, const uint64_t* alloc_mask,
+                      const uint64_t* dirty_mask, const uint8_t* bricks, hipStream_t s, const char* entry) {
+    const std::string name(entry);
+    if (!w) return fail(VRT_ERR_INVALID_ARGUMENT, name + ": null world");
+    int rc = check_world(w, entry);
+    if (rc != VRT_OK) return rc;
+    if (nsec == 0) return VRT_OK;
+    if (!sector_idx || !alloc_mask) return fail(VRT_ERR_INVALID_ARGUMENT, name + ": null sector arrays");
+    if (fault_injected("vrt_world_sync")) return fail(VRT_ERR_HIP, name + ": injected HIP fault");
+    {
+        std::vector<uint32_t> sorted(sector_idx, sector_idx + nsec);
+        std::sort(sorted.begin(), sorted.end());
+        if (sorted.back() >= w->nsec)
+            return fail(VRT_ERR_INVALID_ARGUMENT, name + ": sector index " + std::to_string(sorted.back()) +
+                                                      " out of range (" + std::to_string(w->nsec) + " sectors)");
+        for (uint32_t i = 1; i < nsec; i++)
+            if (sorted[i] == sorted[i - 1])
+                return fail(VRT_ERR_INVALID_ARGUMENT, name + ": sector " + std::to_string(sorted[i]) + " listed twice");
+    }
+    // allocation on a copy of the arena, committed only when every sector fits
+    SlotArena arena = w->arena;
+    std::vector<uint32_t> new_base(nsec);
+    uint64_t nup = 0;
+    for (uint32_t i = 0; i < nsec; i++) {
+        uint32_t idx = sector_idx[i];
+        uint64_t old_m = w->h_mask[idx], new_m = alloc_mask[i];
+        uint64_t dirty = dirty_mask ? dirty_mask[i] & new_m : new_m;
+        if (uint64_t fresh = new_m & ~old_m & ~dirty)
+            return fail(VRT_ERR_INVALID_ARGUMENT, name + ": brick " + std::to_string(std::countr_zero(fresh)) +
+                                                      " of sector " + std::to_string(idx) +
+                                                      " is newly allocated but not dirty (its bytes are needed)");
+        uint32_t base = w->h_base[idx];
+        if (!arena.realloc(base, (uint32_t)std::popcount(old_m), (uint32_t)std::popcount(new_m)))
+            return fail(VRT_ERR_CAPACITY, "Could not allocate brick slots (sector " + std::to_string(idx) + ", " +
+                                              std::to_string(std::popcount(new_m)) + " bricks, " +
+                                              std::to_string(arena.allocated()) + "/" +
+                                              std::to_string(arena.capacity()) + " slots in use)");
+        new_base[i] = new_m ? base : 0u;
+        nup += (uint64_t)std::popcount(dirty);
+    }
+    if (nup && !bricks) return fail(VRT_ERR_INVALID_ARGUMENT, name + ": null brick data");
+    if (nup > 0xFFFFFFFFull) return fail(VRT_ERR_INVALID_ARGUMENT, name + ": too many bricks in one call");
+    // destination slot of every uploaded brick (in the order of `bricks`), and the kept bricks
+    // whose slot changes
+    std::vector<uint32_t> up_slot;
+    up_slot.reserve((size_t)nup);
+    std::vector<uint32_t> mv_src, mv_dst;
+    for (uint32_t i = 0; i < nsec; i++) {
+        uint32_t idx = sector_idx[i];
+        uint64_t old_m = w->h_mask[idx], new_m = alloc_mask[i];
+        uint64_t dirty = dirty_mask ? dirty_mask[i] & new_m : new_m;
+        for (uint64_t m = new_m; m; m &= m - 1) {
+            uint32_t b = (uint32_t)std::countr_zero(m);
+            uint64_t below = (1ull << b) - 1ull;
+            uint32_t slot = new_base[i] + (uint32_t)std::popcount(new_m & below);
+            if ((dirty >> b) & 1ull) {
+                up_slot.push_back(slot);
+            } else {
+                uint32_t old_slot = w->h_base[idx] + (uint32_t)std::popcount(old_m & below);
+                if (old_slot != slot) {
+                    mv_src.push_back(old_slot);
+                    mv_dst.push_back(slot);
+                }
+            }
+        }
+    }
+    const uint32_t nmv = (uint32_t)mv_src.size();
+    // sector records and the sector-of-sectors words they change
+    std::vector<uint4> recs(nsec);
+    std::vector<uint32_t> rec_idx(sector_idx, sector_idx + nsec);
+    std::map<uint32_t, uint64_t> groups;
+    {
+        uint32_t nxz = 1u << w->shift_xz;
+        uint32_t gm_xz = (1u << w->gshift_xz) - 1u, gm_y = (1u << w->gshift_y) - 1u;
+        for (uint32_t i = 0; i < nsec; i++) {
+            uint32_t idx = sector_idx[i];
+            recs[i] = sector_record(alloc_mask[i], new_base[i]);
+            uint32_t sx = idx & (nxz - 1u), sz = (idx >> w->shift_xz) & (nxz - 1u), sy = idx >> (2 * w->shift_xz);
+            uint32_t g = ((sx >> 2) & gm_xz) | ((sz >> 2) & gm_xz) << w->gshift_xz |
+                         ((sy >> 2) & gm_y) << (2 * w->gshift_xz);
+            uint64_t bit = 1ull << ((sx & 3u) | (sz & 3u) << 2 | (sy & 3u) << 4);
+            auto it = groups.emplace(g, w->h_groups[g]).first;
+            it->second = alloc_mask[i] ? (it->second | bit) : (it->second & ~bit);
+        }
+    }
+    // every allocation before any device work: slot growth, staging, the move area
+    rc = grow_slots(w, arena.high_water(), s);
+    if (rc == VRT_OK) rc = ensure_staging(w);
+    if (rc == VRT_OK) rc = ensure_moves(w, nmv);
+    if (rc != VRT_OK) return rc;
+
+    // device work; a failure from here on leaves the device tables part-written
+    auto device_fail = [&](hipError_t e, const char* what) -> int {
+        w->failed = true;
+        hip_fail(e, what);
+        return fail(VRT_ERR_HIP, name + ": " + vrt_last_error() + "; the world is unusable");
+    };
+    hipError_t e = hipSuccess;
+    if (nmv) {
+        // gather every moved brick before any slot is written, then place them
+        if (w->moves_pending) e = hipStreamWaitEvent(s, w->moves_ev, 0);
+        std::vector<uint32_t> lists(mv_src);
+        lists.insert(lists.end(), mv_dst.begin(), mv_dst.end());
+        if (e == hipSuccess) e = stream_to_device(w, s, w->d_moves, (const uint8_t*)lists.data(), lists.size() * 4);
+        if (e != hipSuccess) return device_fail(e, "moving kept bricks");
+        const uint32_t* d_src = (const uint32_t*)w->d_moves;
+        const uint32_t* d_dst = d_src + nmv;
+        uint8_t* area = w->d_moves + (((size_t)nmv * 8 + 255) & ~(size_t)255);
+        uint32_t blocks = (uint32_t)(((uint64_t)nmv * 9 + 255) / 256);
+        sync_gather_kernel<<<blocks, 256, 0, s>>>(w->d_vox, w->d_occ, d_src, nmv, area);
+        sync_place_kernel<<<blocks, 256, 0, s>>>(w->d_vox, w->d_occ, d_dst, nmv, area);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipEventRecord(w->moves_ev, s);
+        if (e != hipSuccess) return device_fail(e, "moving kept bricks");
+        w->moves_pending = true;
+    }
+    // uploaded bricks, a staging half at a time: the bytes, then their slots
+    const uint64_t per_chunk = kSyncHalf / (512 + 4);
+    for (uint64_t k0 = 0; k0 < nup; k0 += per_chunk) {
+        uint32_t cnt = (uint32_t)std::min<uint64_t>(per_chunk, nup - k0);
+        uint32_t h;
+        e = acquire_half(w, h);
+        if (e != hipSuccess) return device_fail(e, "uploading bricks");
+        uint8_t* hp = w->h_stage + h * kSyncHalf;
+        uint8_t* dp = w->d_stage + h * kSyncHalf;
+        size_t bytes_b = (size_t)cnt * 512;
+        std::memcpy(hp, bricks + k0 * 512, bytes_b);
+        std::memcpy(hp + bytes_b, up_slot.data() + k0, (size_t)cnt * 4);
+        size_t bytes = bytes_b + (size_t)cnt * 4;
+        e = hipMemcpyAsync(dp, hp, bytes, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) {
+            sync_upload_kernel<<<(uint32_t)(((uint64_t)cnt * 8 + 255) / 256), 256, 0, s>>>(
+                w->d_vox, w->d_occ, dp, (const uint32_t*)(dp + bytes_b), cnt);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = release_half(w, h, s);
+        if (e != hipSuccess) return device_fail(e, "uploading bricks");
+        w->upload_bytes += bytes;
+        w->upload_brick_bytes += bytes_b;
+    }
+    if (fault_injected("vrt_world_sync:device")) return device_fail(hipErrorUnknown, "injected device failure");
+    // sector records, then the changed sector-of-sectors words, a staging half at a time
+    std::vector<uint64_t> gval;
+    std::vector<uint32_t> gidx;
+    for (auto& [g, v] : groups) {
+        gidx.push_back(g);
+        gval.push_back(v);
+    }
+    const uint64_t nrec = nsec, ngrp = gidx.size();
+    for (uint64_t r0 = 0, g0 = 0; r0 < nrec || g0 < ngrp;) {
+        uint32_t nr = (uint32_t)std::min<uint64_t>(nrec - r0, (kSyncHalf - 256) / 20);
+        uint32_t ng = nr ? 0u : (uint32_t)std::min<uint64_t>(ngrp - g0, (kSyncHalf - 256) / 12);
+        uint32_t h;
+        e = acquire_half(w, h);
+        if (e != hipSuccess) return device_fail(e, "sector records");
+        uint8_t* hp = w->h_stage + h * kSyncHalf;
+        uint8_t* dp = w->d_stage + h * kSyncHalf;
+        size_t o_idx = align16((size_t)nr * 16), o_gval = align16(o_idx + (size_t)nr * 4);
+        size_t o_gidx = o_gval + (size_t)ng * 8, bytes = o_gidx + (size_t)ng * 4;
+        std::memcpy(hp, recs.data() + r0, (size_t)nr * 16);
+        std::memcpy(hp + o_idx, rec_idx.data() + r0, (size_t)nr * 4);
+        std::memcpy(hp + o_gval, gval.data() + g0, (size_t)ng * 8);
+        std::memcpy(hp + o_gidx, gidx.data() + g0, (size_t)ng * 4);
+        e = hipMemcpyAsync(dp, hp, bytes, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) {
+            sync_records_kernel<<<(nr + ng + 255) / 256, 256, 0, s>>>(
+                w->d_sec, (const uint4*)dp, (const uint32_t*)(dp + o_idx), nr, w->d_groups,
+                (const uint64_t*)(dp + o_gval), (const uint32_t*)(dp + o_gidx), ng);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = release_half(w, h, s);
+        if (e != hipSuccess) return device_fail(e, "sector records");
+        w->upload_bytes += bytes;
+        r0 += nr;
+        g0 += ng;
+    }
+    // commit the host mirror
+    w->arena = arena;
+    for (uint32_t i = 0; i < nsec; i++) {
+        uint32_t idx = sector_idx[i];
+        w->num_bricks -= (uint32_t)std::popcount(w->h_mask[idx]);
+        w->num_bricks += (uint32_t)std::popcount(alloc_mask[i]);
+        w->h_mask[idx] = alloc_mask[i];
+        w->h_base[idx] = new_base[i];
+    }
+    for (size_t j = 0; j < gidx.size(); j++) w->h_groups[gidx[j]] = gval[j];
+    w->moved_bricks += nmv;
+    return VRT_OK;
 }
 
 // ---------------------------------------------------------------- C ABI
@@ -315,8 +640,13 @@ int vrt_world_get_info(const vrt_world* w, vrt_world_info* out) {
         out->num_bricks = w->num_bricks;
         out->slot_capacity = w->slot_cap;
         out->device_bytes = (uint64_t)w->nsec * 16 + 256 * 8 + (uint64_t)w->slot_cap * (512 + 64) +
-                            (uint64_t)w->h_groups.size() * 8 + 512 * 8;
+                            (uint64_t)w->h_groups.size() * 8 + 512 * 8 + (w->d_stage ? 2 * kSyncHalf : 0) +
+                            w->moves_cap;
         out->slot_high_water = w->arena.high_water();
+        out->generation = w->generation;
+        out->upload_bytes = w->upload_bytes;
+        out->upload_brick_bytes = w->upload_brick_bytes;
+        out->moved_bricks = w->moved_bricks;
         return VRT_OK;
     });
 }
@@ -335,6 +665,8 @@ int vrt_world_get_buffers(const vrt_world* w, vrt_world_buffers* out) {
 int vrt_world_set_palette(vrt_world* w, const uint64_t* palette, void* stream) {
     return guarded([&]() -> int {
         if (!w || !palette) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_world_set_palette: null argument");
+        int rc = check_world(w, "vrt_world_set_palette");
+        if (rc != VRT_OK) return rc;
         hipStream_t s = (hipStream_t)stream;
         VRT_HIP(hipMemcpyAsync(w->d_palette, palette, 256 * 8, hipMemcpyHostToDevice, s));
         VRT_HIP(hipStreamSynchronize(s));
@@ -345,117 +677,16 @@ int vrt_world_set_palette(vrt_world* w, const uint64_t* palette, void* stream) {
 int vrt_world_sync(vrt_world* w, uint32_t nsec, const uint32_t* sector_idx, const uint64_t* alloc_mask,
                    const uint8_t* bricks, void* stream) {
     return guarded([&]() -> int {
-        if (!w) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_world_sync: null world");
-        if (nsec == 0) return VRT_OK;
-        if (!sector_idx || !alloc_mask) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_world_sync: null sector arrays");
-        if (fault_injected("vrt_world_sync")) return fail(VRT_ERR_HIP, "vrt_world_sync: injected HIP fault");
-        hipStream_t s = (hipStream_t)stream;
-        {
-            std::vector<uint32_t> sorted(sector_idx, sector_idx + nsec);
-            std::sort(sorted.begin(), sorted.end());
-            if (sorted.back() >= w->nsec)
-                return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_world_sync: sector index " + std::to_string(sorted.back()) +
-                                                          " out of range (" + std::to_string(w->nsec) + " sectors)");
-            for (uint32_t i = 1; i < nsec; i++)
-                if (sorted[i] == sorted[i - 1])
-                    return fail(VRT_ERR_INVALID_ARGUMENT,
-                                "vrt_world_sync: sector " + std::to_string(sorted[i]) + " listed twice");
-        }
-        // Allocation pass on copies, committed only when every sector fits.
-        SlotArena arena = w->arena;
-        struct Run {
-            uint32_t dst, src, count;
-        };
-        std::vector<Run> runs;
-        std::vector<uint32_t> new_base(nsec);
-        uint64_t total = 0;
-        for (uint32_t i = 0; i < nsec; i++) {
-            uint32_t idx = sector_idx[i];
-            uint32_t cnt_old = (uint32_t)std::popcount(w->h_mask[idx]);
-            uint32_t cnt_new = (uint32_t)std::popcount(alloc_mask[i]);
-            uint32_t base = w->h_base[idx];
-            if (!arena.realloc(base, cnt_old, cnt_new))
-                return fail(VRT_ERR_CAPACITY, "Could not allocate brick slots (sector " + std::to_string(idx) + ", " +
-                                                  std::to_string(cnt_new) + " bricks, " +
-                                                  std::to_string(arena.allocated()) + "/" +
-                                                  std::to_string(arena.capacity()) + " slots in use)");
-            if (cnt_new == 0) base = 0;
-            new_base[i] = base;
-            if (cnt_new) {
-                if (!runs.empty() && runs.back().dst + runs.back().count == base &&
-                    runs.back().src + runs.back().count == total)
-                    runs.back().count += cnt_new;
-                else
-                    runs.push_back({base, (uint32_t)total, cnt_new});
-            }
-            total += cnt_new;
-        }
-        if (total && !bricks) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_world_sync: null brick data");
-        if (total > 0xFFFFFFFFull) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_world_sync: too many bricks in one call");
-        int rc = grow_slots(w, arena.high_water(), s);
-        if (rc != VRT_OK) return rc;
-        // commit host state
-        w->arena = arena;
-        std::vector<uint32_t> slots;
-        slots.reserve((size_t)total);
-        for (uint32_t i = 0; i < nsec; i++) {
-            uint32_t idx = sector_idx[i];
-            w->num_bricks -= (uint32_t)std::popcount(w->h_mask[idx]);
-            w->num_bricks += (uint32_t)std::popcount(alloc_mask[i]);
-            w->h_mask[idx] = alloc_mask[i];
-            w->h_base[idx] = new_base[i];
-            update_group_bit(w, idx);
-        }
-        for (const Run& r : runs)
-            for (uint32_t k = 0; k < r.count; k++) slots.push_back(r.dst + k);
-        // device: bricks, occupancy, headers (in stream order).  One scratch allocation holds
-        // the slots whose occupancy is rebuilt and, when the synced sectors are few against the
-        // grid, their records for a scatter launch (the largest grid's table is 2 GiB).
-        const bool scatter = (uint64_t)nsec * 64 <= w->nsec;
-        const size_t slots_bytes = (slots.size() * 4 + 15) & ~(size_t)15;
-        const size_t tmp_bytes = slots_bytes + (scatter ? (size_t)nsec * 20 : 0);
-        std::vector<uint8_t> h_tmp(tmp_bytes);
-        std::memcpy(h_tmp.data(), slots.data(), slots.size() * 4);
-        if (scatter) {
-            uint4* rec = (uint4*)(h_tmp.data() + slots_bytes);
-            uint32_t* idx = (uint32_t*)(h_tmp.data() + slots_bytes + (size_t)nsec * 16);
-            for (uint32_t i = 0; i < nsec; i++) {
-                idx[i] = sector_idx[i];
-                rec[i] = sector_record(w->h_mask[sector_idx[i]], w->h_base[sector_idx[i]]);
-            }
-        }
-        uint8_t* d_tmp = nullptr;
-        if (tmp_bytes) VRT_HIP(hipMalloc(&d_tmp, tmp_bytes));
-        auto done = [&](hipError_t e, const char* what) -> int {
-            hipError_t e2 = hipStreamSynchronize(s);  // the scratch is in use until the stream gets here
-            (void)hipFree(d_tmp);
-            if (e == hipSuccess) e = e2;
-            return e == hipSuccess ? VRT_OK : hip_fail(e, what);
-        };
-        std::vector<uint4> recs;  // the whole table, when it is uploaded: alive until done()
-        hipError_t e = hipSuccess;
-        for (const Run& r : runs)
-            if (e == hipSuccess)
-                e = hipMemcpyAsync(w->d_vox + (size_t)r.dst * 512, bricks + (size_t)r.src * 512, (size_t)r.count * 512,
-                                   hipMemcpyHostToDevice, s);
-        if (e == hipSuccess && tmp_bytes) e = hipMemcpyAsync(d_tmp, h_tmp.data(), tmp_bytes, hipMemcpyHostToDevice, s);
-        if (e == hipSuccess && !slots.empty())
-            e = launch_build_occupancy(w, (const uint32_t*)d_tmp, (uint32_t)slots.size(), s);
-        if (e != hipSuccess) return done(e, "vrt_world_sync: bricks and occupancy");
-        if (scatter) {
-            scatter_records_kernel<<<(nsec + 255) / 256, 256, 0, s>>>(
-                w->d_sec, (const uint4*)(d_tmp + slots_bytes), (const uint32_t*)(d_tmp + slots_bytes + (size_t)nsec * 16),
-                nsec);
-            e = hipGetLastError();
-        } else {
-            recs.resize(w->nsec);
-            for (uint32_t i = 0; i < w->nsec; i++)
-                recs[i] = sector_record(w->h_mask[i], w->h_base[i]);
-            e = hipMemcpyAsync(w->d_sec, recs.data(), (size_t)w->nsec * 16, hipMemcpyHostToDevice, s);
-        }
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(w->d_groups, w->h_groups.data(), w->h_groups.size() * 8, hipMemcpyHostToDevice, s);
-        return done(e, "vrt_world_sync: sector records");
+        return world_sync(w, nsec, sector_idx, alloc_mask, nullptr, bricks, (hipStream_t)stream, "vrt_world_sync");
+    });
+}
+
+int vrt_world_sync_bricks(vrt_world* w, uint32_t nsec, const uint32_t* sector_idx, const uint64_t* alloc_mask,
+                          const uint64_t* dirty_mask, const uint8_t* bricks, void* stream) {
+    return guarded([&]() -> int {
+        if (nsec && !dirty_mask) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_world_sync_bricks: null dirty masks");
+        return world_sync(w, nsec, sector_idx, alloc_mask, dirty_mask, bricks, (hipStream_t)stream,
+                          "vrt_world_sync_bricks");
     });
 }
 
