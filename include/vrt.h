@@ -73,7 +73,9 @@ typedef struct vrt_world_desc {
     uint32_t log2_size_xz;   /* voxel extent along x and z is 1 << log2_size_xz, 5..14 */
     uint32_t log2_size_y;    /* voxel extent along y is 1 << log2_size_y, 5..14 */
     uint32_t initial_slots;  /* device brick slots reserved up front (0 = grow on demand) */
-    uint32_t max_slots;      /* slot arena capacity (0 = 64 per sector, i.e. never exhausted); at most 2^28 */
+    uint32_t max_slots;      /* slot arena capacity (0 = 128 per sector: twice the bricks the grid holds, so a
+                                sector's run of up to 64 slots can move even in a nearly full world; device
+                                memory follows the highest slot in use, not this bound); at most 2^28 */
 } vrt_world_desc;
 
 /* Shape of a created world (all derived from the descriptor). */
@@ -303,6 +305,7 @@ int vrt_bounce_generate(const vrt_rays* primary, const vrt_hits* hits, uint32_t 
                         uint32_t seed, vrt_ray_buffers* out, uint32_t* src, uint32_t* count,
                         void* scratch, void* stream);
 
+
 /* ---------------------------------------------------------------- cameras */
 
 /*
@@ -474,10 +477,34 @@ int vrt_render_fwd(const vrt_world* world, const vrt_camera* cameras, uint32_t n
 
 /* grad_albedo[v][c] += sum over pixels hitting v of grad_image[p][c] * saved_weight[p].
  * grad_albedo has the albedo's length; the world is unchanged since the forward pass.
- * npixels is at most 2^32 - 256. */
+ * npixels is at most 2^32 - 256.  A scatter-add with wave-aggregated float atomics. */
 int vrt_render_bwd(const vrt_world* world, uint32_t npixels, const float* grad_image,
                    const uint32_t* saved_voxel, const float* saved_weight,
                    float* grad_albedo, void* stream);
+
+/*
+ * The same gradient, every one of the nvox * 3 entries written (no zeroing beforehand):
+ * grad_albedo[v][c] = sum over pixels hitting v of grad_image[p][c] * saved_weight[p], 0 for
+ * voxels no pixel hit.  Hit pixels are bucketed by brick slot (a counting sort), then each
+ * slot's contributions are summed in LDS and its 512 rows written once; no global float
+ * atomics.  nvox >= vrt_world_info.slot_high_water * 512 (the albedo's length); grad_albedo
+ * 16-byte aligned; workspace (256-byte aligned) of vrt_render_bwd_workspace_bytes(npixels,
+ * nvox) bytes.  The summation order within a voxel is not fixed (LDS atomics).
+ */
+size_t vrt_render_bwd_workspace_bytes(uint32_t npixels, uint64_t nvox);
+int vrt_render_bwd_bricks(const vrt_world* world, uint32_t npixels, const float* grad_image,
+                          const uint32_t* saved_voxel, const float* saved_weight, uint64_t nvox,
+                          float* grad_albedo, void* workspace, size_t workspace_bytes, void* stream);
+
+/*
+ * The gradient as npixels unsummed entries (a sparse COO gradient with repeated indices):
+ * index[p] = the voxel pixel p hit and values[p][c] = grad_image[p][c] * saved_weight[p], or
+ * index 0 and values 0 for a pixel that contributes nothing (a miss or zero weight).  Summing
+ * the values by index gives vrt_render_bwd's gradient.
+ */
+int vrt_render_bwd_sparse(const vrt_world* world, uint32_t npixels, const float* grad_image,
+                          const uint32_t* saved_voxel, const float* saved_weight, int64_t* index,
+                          float* values, void* stream);
 
 #ifdef __cplusplus
 }

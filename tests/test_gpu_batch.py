@@ -206,23 +206,24 @@ def test_graph_recaptured_after_slot_growth(vrt, oracle_mod, cuda):
     """A sync that outgrows the slot buffers moves them (the world's generation changes): the next
     graph replay captures again with the new buffers and equals an eager launch and the oracle."""
     import voxelrt_amd as V
-    from helpers import blob_world
     from test_gpu_world import sector_stream_of
     from test_oracle import dense_world
 
-    vox = blob_world(128, seed=5, density=0.002)
+    vox = np.zeros((128, 128, 128), np.uint8)
+    vox[60:70, 40:50, 50:60] = 3  # a few bricks: the slot buffers start at their minimum, 1024 slots
     world = V.World(7, 7)
     world.sync(*sector_stream_of(vox, list(range(64))))
     i0 = world.info()
+    assert i0.slot_capacity == 1024
     cams = [V.CameraSpec((64.5, 100.25, 10.75), 20.0, -35.0, 64, 64),
             V.CameraSpec((20.5, 90.5, 100.5), 200.0, -50.0, 128, 64, anchor="sector")]
     batch = V.Batch(cams, max_iters=128)
     plan = V.BatchPlan(world, batch, (8, 8, 1, 0))
     plan.graph_launch()
     torch.cuda.synchronize()
-    # fill the lower half of the world: many more bricks than the slot buffers hold
+    # a solid floor 48 voxels deep: 16 x 16 x 6 = 1536 more bricks than the slot buffers hold
     vox2 = vox.copy()
-    vox2[:60] = np.where(vox2[:60] == 0, 4, vox2[:60])
+    vox2[:48] = 4
     world.sync(*sector_stream_of(vox2, list(range(64))))
     i1 = world.info()
     assert i1.slot_capacity > i0.slot_capacity and i1.generation != i0.generation

@@ -76,13 +76,26 @@ def test_forward_matches_oracle(vrt, oracle_mod, cuda, c5_small, anchor):
     check_forward(V, W, oracle_mod, world, scene, cams, albedo)
 
 
-def test_backward_matches_reference_and_finite_differences(vrt, oracle_mod, cuda, c5_small):
+def dense_gradient(V, world, grad_image, sv, sw, nvox, mode):
+    """The albedo gradient of one backward form, as float64 [nvox, 3] on the host."""
+    out = V.render_backward(world, grad_image, sv, sw, nvox, mode=mode)
+    torch.cuda.synchronize()
+    if mode != "sparse":
+        return out.cpu().numpy().astype(np.float64)
+    index, values = out
+    dense = np.zeros((nvox, 3), np.float64)
+    np.add.at(dense, index.cpu().numpy(), values.cpu().numpy().astype(np.float64))
+    return dense
+
+
+@pytest.mark.parametrize("mode", ["atomic", "bricks", "sparse"])
+def test_backward_matches_reference_and_finite_differences(vrt, oracle_mod, cuda, c5_small, mode):
     """Tolerance 1e-4 relative to the sum of |terms| of each voxel's gradient: the scale of the
-    rounding error of the fp32 atomics (and of the fp32 image the loss is taken on)."""
-    check_backward(oracle_mod, *c5_small)
+    rounding error of fp32 sums in any order (and of the fp32 image the loss is taken on)."""
+    check_backward(oracle_mod, *c5_small, mode=mode)
 
 
-def check_backward(oracle_mod, world, scene, cams, albedo, nvox):
+def check_backward(oracle_mod, world, scene, cams, albedo, nvox, mode="bricks"):
     import voxelrt_amd as V
     from voxelrt_amd import workloads as W
 
@@ -91,9 +104,7 @@ def check_backward(oracle_mod, world, scene, cams, albedo, nvox):
     target, _, _ = V.render_saved(world, target_albedo, cams, W.MAX_ITERS)
     image, sv, sw = V.render_saved(world, albedo, cams, W.MAX_ITERS)
     grad_image = 2.0 * (image - target)
-    grad = V.render_backward(world, grad_image, sv, sw, nvox)
-    torch.cuda.synchronize()
-    grad = grad.cpu().numpy().astype(np.float64)
+    grad = dense_gradient(V, world, grad_image, sv, sw, nvox, mode)
     sv_h, sw_h = sv.cpu().numpy().view(np.uint32), sw.cpu().numpy()
     alb_h, tgt_h = albedo.cpu().numpy(), target.cpu().numpy()
     ref, abssum = oracle_mod.render_backward(sv_h, sw_h, grad_image.cpu().numpy(), nvox)
@@ -153,29 +164,63 @@ def test_full_configuration_forward_matches_oracle(vrt, oracle_mod, cuda, c5_ful
     assert check_forward(V, W, oracle_mod, world, scene, cams, albedo) == 16 * 512 * 512
 
 
-def test_full_configuration_backward_matches_reference(vrt, oracle_mod, cuda, c5_full):
+@pytest.mark.parametrize("mode", ["bricks", "sparse"])
+def test_full_configuration_backward_matches_reference(vrt, oracle_mod, cuda, c5_full, mode):
     """16 x 512^2 pixels into 49.9 M albedo voxels: the gradient against the FP64 CPU reference
     gradient and FP64 central differences on 256 voxels, as for the small views."""
-    check_backward(oracle_mod, *c5_full)
+    check_backward(oracle_mod, *c5_full, mode=mode)
 
 
-def test_autograd_function(vrt, cuda, c5_small):
+@pytest.mark.parametrize("mode", ["bricks", "atomic", "sparse"])
+def test_autograd_function(vrt, cuda, c5_small, mode):
     import voxelrt_amd as V
     from voxelrt_amd import workloads as W
 
     world, scene, cams, albedo, nvox = c5_small
     target, _, _ = V.render_saved(world, (albedo * 0.9).contiguous(), cams, W.MAX_ITERS)
     a = albedo.clone().requires_grad_(True)
-    image = V.render(world, a, cams, W.MAX_ITERS)
+    image = V.render(world, a, cams, W.MAX_ITERS, grad=mode)
     loss = ((image - target) ** 2).sum()
     loss.backward()
     image2, sv, sw = V.render_saved(world, albedo, cams, W.MAX_ITERS)
     assert torch.equal(image.detach(), image2)
     manual = V.render_backward(world, 2.0 * (image2 - target), sv, sw, nvox)
     torch.cuda.synchronize()
-    err = (a.grad - manual).abs().max().item()
-    assert err <= 1e-4 * manual.abs().max().item()
     assert a.grad.shape == albedo.shape
+    assert a.grad.is_sparse == (mode == "sparse")
+    got = a.grad.to_dense() if mode == "sparse" else a.grad
+    err = (got - manual).abs().max().item()
+    assert err <= 1e-4 * manual.abs().max().item()
+
+
+def test_bricks_backward_writes_every_row(vrt, cuda, c5_small):
+    """The written-whole gradient over an albedo longer than the slots (a partial last brick):
+    rows no pixel hit are 0 whatever the buffer held, the rest equal the atomic form."""
+    import voxelrt_amd as V
+    from voxelrt_amd import workloads as W
+
+    world, scene, cams, albedo, nvox = c5_small
+    n2 = nvox + 700
+    alb = torch.cat([albedo, torch.rand((700, 3), device="cuda")]).contiguous()
+    image, sv, sw = V.render_saved(world, alb, cams[:2], W.MAX_ITERS)
+    gi = torch.randn(image.shape, device="cuda")
+    ref = V.render_backward(world, gi, sv, sw, n2, mode="atomic")
+    grad = torch.empty((n2, 3), device="cuda").fill_(float("nan"))
+    nbytes = int(V.lib().vrt_render_bwd_workspace_bytes(sv.numel(), n2))
+    ws = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    from voxelrt_amd import _lib as L
+
+    L.check(V.lib().vrt_render_bwd_bricks(world.handle, sv.numel(), gi.data_ptr(), sv.data_ptr(), sw.data_ptr(), n2,
+                                          grad.data_ptr(), ws.data_ptr(), nbytes, None))
+    torch.cuda.synchronize()
+    assert not bool(torch.isnan(grad).any())
+    touched = torch.zeros(n2, dtype=torch.bool, device="cuda")
+    touched[sv[(sv >= 0) & (sw != 0)].long()] = True
+    assert bool((grad[~touched] == 0).all())
+    assert float((grad - ref).abs().max()) <= 1e-4 * float(ref.abs().max())
+    with pytest.raises(V.VrtError):  # shorter than the slots the world holds
+        L.check(V.lib().vrt_render_bwd_bricks(world.handle, sv.numel(), gi.data_ptr(), sv.data_ptr(), sw.data_ptr(),
+                                              nvox - 512, grad.data_ptr(), ws.data_ptr(), nbytes, None))
 
 
 def test_forward_many_views_one_launch(vrt, cuda, c5_small):

@@ -295,7 +295,7 @@ def test_brick_sync_uploads_only_dirty_bricks(vrt, oracle_mod, cuda):
     rng = np.random.default_rng(17)
     vox = blob_world(128, seed=41, density=0.02)
     vox[:40] = np.where(vox[:40] == 0, 7, vox[:40])  # solid floor: sectors of the bottom layer are full
-    vox[20:28, 8:16, 72:80] = 0                       # one empty brick inside a full sector
+    vox[16:24, 8:16, 72:80] = 0                       # one empty brick (37 of sector 2) inside a full sector
     nsec = 64
     world = V.World(7, 7)
     world.sync(*sector_stream_of(vox, list(range(nsec))))
@@ -314,6 +314,7 @@ def test_brick_sync_uploads_only_dirty_bricks(vrt, oracle_mod, cuda):
     check("initial")
     full_si, _ = brick_of(4, 4, 4)
     assert brick_alloc(vox, full_si)[0] == (1 << 64) - 1
+    assert brick_alloc(vox, 2)[0] == ((1 << 64) - 1) & ~(1 << 37)
 
     def sync(dirty, tag, bricks_up, moved):
         nonlocal vox

@@ -185,6 +185,9 @@ _SIGNATURES = {
     "vrt_batch_autotune": (C.c_int, [P, C.POINTER(Batch), C.c_char_p, C.POINTER(BatchTuning), C.POINTER(f32), P]),
     "vrt_render_fwd": (C.c_int, [P, C.POINTER(Camera), u32, u32, C.POINTER(f32 * 3), f32, P, P, P, P, P]),
     "vrt_render_bwd": (C.c_int, [P, u32, P, P, P, P, P]),
+    "vrt_render_bwd_workspace_bytes": (C.c_size_t, [u32, u64]),
+    "vrt_render_bwd_bricks": (C.c_int, [P, u32, P, P, P, u64, P, P, C.c_size_t, P]),
+    "vrt_render_bwd_sparse": (C.c_int, [P, u32, P, P, P, P, P, P]),
     # vrt_scene.h
     "vrt_scene_terrain": (C.c_int, [C.POINTER(TerrainParams), u32, C.POINTER(P)]),
     "vrt_scene_spheres": (C.c_int, [u32, u32, u32, f32, f32, u64, u32, C.POINTER(P)]),

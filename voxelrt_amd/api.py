@@ -569,13 +569,19 @@ def _pack_cameras(cameras) -> torch.Tensor:
     return torch.frombuffer(bytearray(raw), dtype=torch.uint8)
 
 
+GRAD_MODES = {"bricks": 0, "atomic": 1, "sparse": 2}
+
+
 def render(world: World, albedo: torch.Tensor, cameras, max_iters: int = 128, light=None,
-           intensity: float = SUN_INTENSITY, stream=None) -> torch.Tensor:
+           intensity: float = SUN_INTENSITY, stream=None, grad: str = "bricks") -> torch.Tensor:
     """Differentiable Lambertian render: [pixels, 3] image, gradients flow to `albedo`
-    ([voxel slots * 512, 3] fp32) through the libtorch autograd Function."""
+    ([voxel slots * 512, 3] fp32) through the libtorch autograd Function.  grad: the albedo
+    gradient's form, "bricks" (dense, written whole: vrt_render_bwd_bricks), "atomic" (dense,
+    zero-filled and scatter-added: vrt_render_bwd) or "sparse" (a COO tensor with one unsummed
+    entry per pixel: vrt_render_bwd_sparse)."""
     light = light if light is not None else sun_direction()
     return torch_ext().render(albedo, world.handle.value, _pack_cameras(cameras), max_iters, list(light),
-                              float(intensity), _stream_handle(stream))
+                              float(intensity), _stream_handle(stream), GRAD_MODES[grad])
 
 
 def render_saved(world: World, albedo: torch.Tensor, cameras, max_iters: int = 128, light=None,
@@ -587,15 +593,31 @@ def render_saved(world: World, albedo: torch.Tensor, cameras, max_iters: int = 1
 
 
 def render_backward(world: World, grad_image: torch.Tensor, saved_voxel: torch.Tensor, saved_weight: torch.Tensor,
-                    nvox: int, stream=None) -> torch.Tensor:
-    """vrt_render_bwd into a fresh zero gradient [nvox, 3]."""
+                    nvox: int, stream=None, mode: str = "atomic"):
+    """The albedo gradient [nvox, 3] from saved hits: mode "atomic" (vrt_render_bwd into a fresh
+    zero gradient), "bricks" (vrt_render_bwd_bricks, written whole) or "sparse"
+    (vrt_render_bwd_sparse: returns (index int64 [n], values [n, 3]), one entry per pixel)."""
     need = world.info().slot_high_water * 512
     if nvox < need:
         raise ValueError(f"render_backward: nvox {nvox} < slot_high_water * 512 = {need}")
     if saved_voxel.numel() and grad_image.numel() != 3 * saved_voxel.numel():
         raise ValueError("render_backward: grad_image must hold 3 values per saved pixel")
-    grad = torch.zeros((nvox, 3), dtype=torch.float32, device=grad_image.device)
     g = grad_image.contiguous()
-    L.check(L.lib().vrt_render_bwd(world.handle, saved_voxel.numel(), g.data_ptr(), saved_voxel.data_ptr(),
-                                   saved_weight.data_ptr(), grad.data_ptr(), _stream_handle(stream)))
+    lib, n, sh = L.lib(), saved_voxel.numel(), _stream_handle(stream)
+    if mode == "sparse":
+        index = torch.empty(n, dtype=torch.int64, device=grad_image.device)
+        values = torch.empty((n, 3), dtype=torch.float32, device=grad_image.device)
+        L.check(lib.vrt_render_bwd_sparse(world.handle, n, g.data_ptr(), saved_voxel.data_ptr(),
+                                          saved_weight.data_ptr(), index.data_ptr(), values.data_ptr(), sh))
+        return index, values
+    if mode == "bricks":
+        grad = torch.empty((nvox, 3), dtype=torch.float32, device=grad_image.device)
+        nbytes = int(lib.vrt_render_bwd_workspace_bytes(n, nvox))
+        ws = torch.empty(nbytes, dtype=torch.uint8, device=grad_image.device)
+        L.check(lib.vrt_render_bwd_bricks(world.handle, n, g.data_ptr(), saved_voxel.data_ptr(), saved_weight.data_ptr(),
+                                          nvox, grad.data_ptr(), ws.data_ptr(), nbytes, sh))
+        return grad
+    grad = torch.zeros((nvox, 3), dtype=torch.float32, device=grad_image.device)
+    L.check(lib.vrt_render_bwd(world.handle, n, g.data_ptr(), saved_voxel.data_ptr(), saved_weight.data_ptr(),
+                               grad.data_ptr(), sh))
     return grad

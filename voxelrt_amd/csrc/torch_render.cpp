@@ -3,9 +3,11 @@
 // The Python module _vrt_torch exposes render(albedo, world, cameras, ...),
 // differentiable in `albedo` (fp32 [num_voxel_slots * 512, 3] on the device,
 // voxel id = slot * 512 + voxel index in the brick).  Forward traces one ray
-// per pixel and shades albedo * max(0, N.L) * intensity; backward scatter-adds
-// dL/dpixel x weight into the albedo gradient.  All device work goes through
-// the C ABI of libvoxelrt_hip.so; torch only owns the tensors.
+// per pixel and shades albedo * max(0, N.L) * intensity; backward sums dL/dpixel x weight
+// per hit voxel into the albedo gradient: a dense gradient written whole (the default), a
+// zero-filled one scatter-added with atomics, or a sparse COO gradient with one entry per
+// pixel.  All device work goes through the C ABI of libvoxelrt_hip.so; torch only owns the
+// tensors.
 #include <torch/extension.h>
 
 #include <cstring>
@@ -43,11 +45,19 @@ void check_inputs(const torch::Tensor& albedo, int64_t world, const std::vector<
                 albedo.size(0));
 }
 
+// How the backward pass forms the albedo gradient.
+enum GradMode : int64_t {
+    GRAD_BRICKS = 0,  // dense, written whole by vrt_render_bwd_bricks (the default)
+    GRAD_ATOMIC = 1,  // dense, zero-filled then scatter-added by vrt_render_bwd
+    GRAD_SPARSE = 2   // sparse COO, one unsummed entry per pixel (vrt_render_bwd_sparse)
+};
+
 class RenderFunction : public torch::autograd::Function<RenderFunction> {
 public:
     static torch::Tensor forward(torch::autograd::AutogradContext* ctx, torch::Tensor albedo, int64_t world,
                                  torch::Tensor cameras, int64_t max_iters, std::vector<double> light,
-                                 double intensity, int64_t stream) {
+                                 double intensity, int64_t stream, int64_t grad_mode) {
+        TORCH_CHECK(grad_mode >= GRAD_BRICKS && grad_mode <= GRAD_SPARSE, "grad_mode: 0 (bricks), 1 (atomic), 2 (sparse)");
         check_inputs(albedo, world, light);
         auto cams = unpack_cameras(cameras);
         int64_t npix = 0;
@@ -66,6 +76,7 @@ public:
         ctx->saved_data["world"] = world;
         ctx->saved_data["stream"] = stream;
         ctx->saved_data["nvox"] = albedo.size(0);
+        ctx->saved_data["mode"] = grad_mode;
         return image;
     }
 
@@ -76,20 +87,42 @@ public:
         int64_t world = ctx->saved_data["world"].toInt();
         int64_t stream = ctx->saved_data["stream"].toInt();
         int64_t nvox = ctx->saved_data["nvox"].toInt();
+        int64_t mode = ctx->saved_data["mode"].toInt();
         auto g = grad_outputs[0].contiguous();
-        auto grad = torch::zeros({nvox, 3}, g.options());
-        check(vrt_render_bwd((const vrt_world*)world, (uint32_t)saved_voxel.numel(), g.data_ptr<float>(),
-                             (const uint32_t*)saved_voxel.data_ptr<int32_t>(), saved_weight.data_ptr<float>(),
-                             grad.data_ptr<float>(), (void*)stream),
-              "vrt_render_bwd");
+        const auto* w = (const vrt_world*)world;
+        const uint32_t np = (uint32_t)saved_voxel.numel();
+        const auto* sv = (const uint32_t*)saved_voxel.data_ptr<int32_t>();
+        torch::Tensor grad;
+        if (mode == GRAD_SPARSE) {
+            auto index = torch::empty({1, (int64_t)np}, g.options().dtype(torch::kInt64));
+            auto values = torch::empty({(int64_t)np, 3}, g.options());
+            check(vrt_render_bwd_sparse(w, np, g.data_ptr<float>(), sv, saved_weight.data_ptr<float>(),
+                                        index.data_ptr<int64_t>(), values.data_ptr<float>(), (void*)stream),
+                  "vrt_render_bwd_sparse");
+            // indices are in range by construction (saved ids < slot_high_water * 512 <= nvox):
+            // no validation pass, which would read them back to the host
+            grad = at::_sparse_coo_tensor_unsafe(index, values, {nvox, 3});
+        } else if (mode == GRAD_ATOMIC) {
+            grad = torch::zeros({nvox, 3}, g.options());
+            check(vrt_render_bwd(w, np, g.data_ptr<float>(), sv, saved_weight.data_ptr<float>(), grad.data_ptr<float>(),
+                                 (void*)stream),
+                  "vrt_render_bwd");
+        } else {
+            grad = torch::empty({nvox, 3}, g.options());
+            size_t bytes = vrt_render_bwd_workspace_bytes(np, (uint64_t)nvox);
+            auto ws = torch::empty({(int64_t)bytes}, g.options().dtype(torch::kUInt8));
+            check(vrt_render_bwd_bricks(w, np, g.data_ptr<float>(), sv, saved_weight.data_ptr<float>(), (uint64_t)nvox,
+                                        grad.data_ptr<float>(), ws.data_ptr(), bytes, (void*)stream),
+                  "vrt_render_bwd_bricks");
+        }
         return {grad, torch::Tensor(), torch::Tensor(), torch::Tensor(), torch::Tensor(), torch::Tensor(),
-                torch::Tensor()};
+                torch::Tensor(), torch::Tensor()};
     }
 };
 
 torch::Tensor render(torch::Tensor albedo, int64_t world, torch::Tensor cameras, int64_t max_iters,
-                     std::vector<double> light, double intensity, int64_t stream) {
-    return RenderFunction::apply(albedo, world, cameras, max_iters, light, intensity, stream);
+                     std::vector<double> light, double intensity, int64_t stream, int64_t grad_mode) {
+    return RenderFunction::apply(albedo, world, cameras, max_iters, light, intensity, stream, grad_mode);
 }
 
 // Forward only, also returning the saved hit voxel ids and shading weights.

@@ -576,7 +576,10 @@ int vrt_world_create(const vrt_world_desc* desc, void* stream, vrt_world** out) 
         w->shift_xz = w->log2_xz - 5;
         w->shift_y = w->log2_y - 5;
         w->nsec = 1u << (2 * w->shift_xz + w->shift_y);
-        uint64_t max_slots = (uint64_t)w->nsec * 64;
+        // the arena bounds slot addresses only (memory follows the high-water mark): by default
+        // twice the bricks the grid can hold, so that a sector whose run must move finds a
+        // contiguous free range even when the world is nearly full
+        uint64_t max_slots = (uint64_t)w->nsec * 128;
         if (desc->max_slots) max_slots = std::min<uint64_t>(max_slots, desc->max_slots);
         // occupancy indices slot * 8 + cell are 32-bit in the traversal: at most 2^28 slots
         // (154 GB of brick data)
