@@ -305,6 +305,20 @@ int vrt_bounce_generate(const vrt_rays* primary, const vrt_hits* hits, uint32_t 
                         uint32_t seed, vrt_ray_buffers* out, uint32_t* src, uint32_t* count,
                         void* scratch, void* stream);
 
+/*
+ * vrt_bounce_generate without a workspace, and allocating nothing: no counterpart in the
+ * reference.  For callers that cannot allocate device memory: code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc.  The per-block hit counts live in a
+ * static device array of 8 KiB (2048 counts, one per 4096 primaries) that every GPU the
+ * library runs on holds until the process ends; its address is looked up once per device, on
+ * the first call with that device current.  At most 8388608 primaries (a 3840x2160 frame
+ * fits); more is VRT_ERR_INVALID_ARGUMENT.  Each call writes every count it reads, so nothing
+ * needs clearing between calls.  All calls on one GPU share the array: calls that overlap in
+ * time (on different streams) race.  Results are identical to vrt_bounce_generate's.
+ */
+int vrt_bounce_generate_noalloc(const vrt_rays* primary, const vrt_hits* hits, uint32_t width,
+                                uint32_t seed, vrt_ray_buffers* out, uint32_t* src, uint32_t* count,
+                                void* stream);
 
 /* ---------------------------------------------------------------- cameras */
 

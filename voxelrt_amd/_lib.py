@@ -173,6 +173,8 @@ _SIGNATURES = {
     "vrt_cast_one_f64": (C.c_int, [P, C.POINTER(C.c_double * 3), C.POINTER(C.c_double * 3), u32, C.POINTER(HitF64)]),
     "vrt_bounce_scratch_bytes": (C.c_size_t, [u32]),
     "vrt_bounce_generate": (C.c_int, [C.POINTER(Rays), C.POINTER(Hits), u32, u32, C.POINTER(RayBuffers), P, P, P, P]),
+    "vrt_bounce_generate_noalloc": (C.c_int, [C.POINTER(Rays), C.POINTER(Hits), u32, u32, C.POINTER(RayBuffers), P, P,
+                                              P]),
     "vrt_camera_rays": (C.c_int, [C.POINTER(Camera), C.POINTER(RayBuffers), P]),
     "vrt_cast_shader": (C.c_int, [P, C.POINTER(Rays), C.POINTER(ShaderOpts), C.POINTER(Hits), P]),
     "vrt_render_frame": (C.c_int, [P, C.POINTER(Camera), C.POINTER(RenderOpts), C.POINTER(GBuffer), P]),

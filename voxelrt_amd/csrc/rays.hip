@@ -10,6 +10,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <mutex>
+#include <string>
 
 #include "vrt_cast_impl.h"
 #include "vrt_math.h"
@@ -237,6 +239,82 @@ __global__ __launch_bounds__(1024) void bounce_write_rows_kernel(const int4* __r
 
 using namespace vrt;
 
+// ---------------------------------------------------------------- no-allocation scratch
+
+// Scratch of vrt_bounce_generate_noalloc: the hit count of each block of kGenBlock primaries
+// of the two-launch generation.  Every GPU the library runs on holds its own copy of this
+// array, at its own address, until the process ends.  In host code the array's name is not a
+// device address: kernels get the address hipGetSymbolAddress returns for the device.
+static __device__ __align__(256) uint32_t scratch[kGenMaxBlocks];
+
+// Primaries the no-allocation entry point takes: kGenMaxBlocks blocks of kGenBlock (8 388 608,
+// e.g. a 3840x2160 frame).
+static const uint32_t kNoAllocMaxPrimaries = kGenMaxBlocks * kGenBlock;
+
+static std::mutex g_scratch_mu;
+static uint32_t* g_scratch_ptr[64];  // per device: the device address of `scratch`
+
+// The device address of `scratch` on the current device, looked up once per device.
+static int scratch_of_current_device(uint32_t** out) {
+    int dev = 0;
+    VRT_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return fail(VRT_ERR_NO_DEVICE, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return hip_fail(e != hipSuccess ? e : hipErrorInvalidSymbol, "hipGetSymbolAddress(scratch)");
+        g_scratch_ptr[dev] = (uint32_t*)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return VRT_OK;
+}
+
+// ---------------------------------------------------------------- bounce generation
+
+// Checks the arguments and launches the generation with `blocks` as the per-block counts
+// (at least vrt_bounce_scratch_bytes(n) bytes, or kGenMaxBlocks counts when n <= max_n is
+// the two-launch range).  Every block count the launches read is written by them first.
+static int bounce_generate(const vrt_rays* primary, const vrt_hits* hits, uint32_t width, uint32_t seed,
+                           vrt_ray_buffers* out, uint32_t* src, uint32_t* count, uint32_t* blocks, uint32_t max_n,
+                           hipStream_t s, const char* entry) {
+    const std::string name(entry);
+    if (!primary || !hits || !out || !count) return fail(VRT_ERR_INVALID_ARGUMENT, name + ": null argument");
+    if (width == 0) return fail(VRT_ERR_INVALID_ARGUMENT, name + ": width must be > 0");
+    uint32_t n = primary->n;
+    // block counts below are 32-bit: (n + 4095) / 4096 and (n + 255) / 256 must not wrap
+    if (n > max_n)
+        return fail(VRT_ERR_INVALID_ARGUMENT,
+                    name + (max_n == kMaxBouncePrimaries ? ": at most 2^32 - 4096 primary rays"
+                                                         : ": at most 8388608 primary rays (the size of its scratch)"));
+    if (n && (!src || !blocks || !hits->rec)) return fail(VRT_ERR_INVALID_ARGUMENT, name + ": null argument");
+    if (n && !hits->pos) return fail(VRT_ERR_INVALID_ARGUMENT, name + ": hits need positions");
+    if (n && (!out->ox || !out->oy || !out->oz || !out->dx || !out->dy || !out->dz))
+        return fail(VRT_ERR_INVALID_ARGUMENT, name + ": null ray buffer");
+    if (n == 0) {
+        // writes *count = 0 and reads no block count
+        hipLaunchKernelGGL(bounce_scan_kernel, dim3(1), dim3(1024), 0, s, blocks, 0u, count);
+        VRT_HIP(hipGetLastError());
+        return VRT_OK;
+    }
+    const int4* rec = (const int4*)hits->rec;
+    const uint32_t nrows = (n + kGenBlock - 1) / kGenBlock;
+    if (nrows <= kGenMaxBlocks) {
+        bounce_count_rows_kernel<<<nrows, 1024, 0, s>>>(rec, n, blocks);
+        bounce_write_rows_kernel<<<nrows, 1024, 0, s>>>(rec, hits->pos, hits->pos + n, hits->pos + 2 * (size_t)n, n,
+                                                        width, seed, blocks, nrows, *out, src, count);
+        VRT_HIP(hipGetLastError());
+        return VRT_OK;
+    }
+    uint32_t nb = (n + kBounceBlock - 1) / kBounceBlock;
+    bounce_count_kernel<<<nb, 256, 0, s>>>(rec, n, blocks);
+    bounce_scan_kernel<<<1, 1024, 0, s>>>(blocks, nb, count);
+    bounce_write_kernel<<<nb, 256, 0, s>>>(rec, hits->pos, hits->pos + n, hits->pos + 2 * (size_t)n, n, width, seed,
+                                           blocks, *out, src);
+    VRT_HIP(hipGetLastError());
+    return VRT_OK;
+}
+
 extern "C" {
 
 int vrt_camera_rays(const vrt_camera* cam, vrt_ray_buffers* out, void* stream) {
@@ -264,42 +342,23 @@ size_t vrt_bounce_scratch_bytes(uint32_t n) {
 }
 
 int vrt_bounce_generate(const vrt_rays* primary, const vrt_hits* hits, uint32_t width, uint32_t seed,
-                        vrt_ray_buffers* out, uint32_t* src, uint32_t* count, void* scratch, void* stream) {
+                        vrt_ray_buffers* out, uint32_t* src, uint32_t* count, void* workspace, void* stream) {
     return guarded([&]() -> int {
-        if (!primary || !hits || !out || !count)
+        if (primary && primary->n && !workspace)
             return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_bounce_generate: null argument");
-        if (width == 0) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_bounce_generate: width must be > 0");
-        hipStream_t s = (hipStream_t)stream;
-        uint32_t n = primary->n;
-        // block counts below are 32-bit: (n + 4095) / 4096 and (n + 255) / 256 must not wrap
-        if (n > kMaxBouncePrimaries)
-            return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_bounce_generate: at most 2^32 - 4096 primary rays");
-        if (n && (!src || !scratch || !hits->rec)) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_bounce_generate: null argument");
-        if (n && !hits->pos) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_bounce_generate: hits need positions");
-        if (n && (!out->ox || !out->oy || !out->oz || !out->dx || !out->dy || !out->dz))
-            return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_bounce_generate: null ray buffer");
-        uint32_t nb = (n + kBounceBlock - 1) / kBounceBlock;
-        uint32_t* blocks = (uint32_t*)scratch;
-        if (n == 0) {
-            hipLaunchKernelGGL(bounce_scan_kernel, dim3(1), dim3(1024), 0, s, blocks, 0u, count);
-            VRT_HIP(hipGetLastError());
-            return VRT_OK;
-        }
-        const int4* rec = (const int4*)hits->rec;
-        const uint32_t nrows = (n + kGenBlock - 1) / kGenBlock;
-        if (nrows <= kGenMaxBlocks) {
-            bounce_count_rows_kernel<<<nrows, 1024, 0, s>>>(rec, n, blocks);
-            bounce_write_rows_kernel<<<nrows, 1024, 0, s>>>(rec, hits->pos, hits->pos + n, hits->pos + 2 * (size_t)n,
-                                                            n, width, seed, blocks, nrows, *out, src, count);
-            VRT_HIP(hipGetLastError());
-            return VRT_OK;
-        }
-        bounce_count_kernel<<<nb, 256, 0, s>>>(rec, n, blocks);
-        bounce_scan_kernel<<<1, 1024, 0, s>>>(blocks, nb, count);
-        bounce_write_kernel<<<nb, 256, 0, s>>>(rec, hits->pos, hits->pos + n, hits->pos + 2 * (size_t)n, n, width,
-                                               seed, blocks, *out, src);
-        VRT_HIP(hipGetLastError());
-        return VRT_OK;
+        return bounce_generate(primary, hits, width, seed, out, src, count, (uint32_t*)workspace, kMaxBouncePrimaries,
+                               (hipStream_t)stream, "vrt_bounce_generate");
+    });
+}
+
+int vrt_bounce_generate_noalloc(const vrt_rays* primary, const vrt_hits* hits, uint32_t width, uint32_t seed,
+                                vrt_ray_buffers* out, uint32_t* src, uint32_t* count, void* stream) {
+    return guarded([&]() -> int {
+        uint32_t* blocks = nullptr;
+        int rc = scratch_of_current_device(&blocks);
+        if (rc != VRT_OK) return rc;
+        return bounce_generate(primary, hits, width, seed, out, src, count, blocks, kNoAllocMaxPrimaries,
+                               (hipStream_t)stream, "vrt_bounce_generate_noalloc");
     });
 }
 
