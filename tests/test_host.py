@@ -144,9 +144,11 @@ def test_oversized_launches_are_rejected_before_any_device_work(vrt):
     if torch.cuda.is_available():
         pytest.skip("a device is present: the pointers below are host placeholders")
     lib = vrt.lib()
-    host = C.create_string_buffer(64)
+    # a zeroed host block larger than a vrt_world: the checks read only its failed-state flag
+    # (false) before rejecting the sizes; no device memory is touched
+    host = C.create_string_buffer(1 << 16)
     p = C.addressof(host)
-    fake_world = C.c_void_p(p)  # never dereferenced on the rejection path
+    fake_world = C.c_void_p(p)
     # render backward: at most 2^32 - 256 pixels
     assert lib.vrt_render_bwd(fake_world, 0xFFFFFFFF, p, p, p, p, None) == L.VRT_ERR_INVALID_ARGUMENT
     assert b"2^32 - 256" in lib.vrt_last_error()
@@ -165,3 +167,4 @@ def test_oversized_launches_are_rejected_before_any_device_work(vrt):
     assert lib.vrt_cast_shader(fake_world, C.byref(rays), C.byref(opts), C.byref(hits), None) == \
         L.VRT_ERR_INVALID_ARGUMENT
     assert b"too many rays" in lib.vrt_last_error()
+
