@@ -184,6 +184,14 @@ typedef struct vrt_hits {
     float* t;        /* [n] distance along d to the entry face; -1 on a miss */
     vrt_hit* rec;    /* [n] */
     float* pos;      /* optional [3 * n] SoA hit position relative to world_origin (NULL = skip) */
+    /* VHitResult's other fields (CpuRenderer.cpp:88-108, 210-221), filled by vrt_cast and
+     * vrt_cast_ex only (the other entry points reject them):
+     *   uv             optional [2 * n] SoA: fract(X ? pos.y : pos.x), fract(Z ? pos.y : pos.z)
+     *                  with X / Z the normal's tie masks and fract(x) = x - floor(x); 0 on a miss
+     *   material_data  optional [n]: the low 32 bits of the hit material's palette entry
+     *                  (vrt_world_set_palette), 0 on a miss */
+    float* uv;
+    uint32_t* material_data;
 } vrt_hits;
 
 /* Totals over a launch, accumulated into device memory when requested. */

@@ -64,10 +64,11 @@ void oracle_world_destroy(oracle_world* w);
 void oracle_world_set_palette(oracle_world* w, const uint64_t* palette);
 uint8_t oracle_world_get_voxel(const oracle_world* w, int32_t x, int32_t y, int32_t z);
 
-/* Scalar cast of rays [begin, end). t[i] = -1 on a miss. pos may be NULL. descents may be NULL. */
+/* Scalar cast of rays [begin, end). t[i] = -1 on a miss. pos, descents and uv (the hit's
+ * texture coordinates, CpuRenderer.cpp:217-220) may be NULL. */
 void oracle_cast(const oracle_world* w, const float* const o[3], const float* const d[3],
                  const int32_t world_origin[3], uint32_t begin, uint32_t end, uint32_t max_iters,
-                 float* t, oracle_hit* rec, float* const pos[3], uint16_t* descents);
+                 float* t, oracle_hit* rec, float* const pos[3], uint16_t* descents, float* const uv[2]);
 
 /* GetStepPos (CpuRenderer.cpp:135-171) alone, for n in-bounds voxel positions v_in[3 * i]
  * and directions d[3 * i]: solid[i] = 1 when the voxel is solid (v_out = v_in), else v_out

@@ -41,7 +41,7 @@ def lib() -> C.CDLL:
         h.oracle_world_get_voxel.argtypes = [P, C.c_int32, C.c_int32, C.c_int32]
         h.oracle_world_get_voxel.restype = C.c_uint8
         h.oracle_cast.argtypes = [P, C.POINTER(P * 3), C.POINTER(P * 3), C.POINTER(C.c_int32 * 3), u32, u32, u32, P,
-                                  P, P, P]
+                                  P, P, P, C.POINTER(P * 2)]
         h.oracle_count.argtypes = [P, C.POINTER(P * 3), C.POINTER(P * 3), C.POINTER(C.c_int32 * 3), u32, u32,
                                    C.POINTER(OracleCounters)]
         h.oracle_cast_simd.argtypes = [P, C.POINTER(P * 3), C.POINTER(P * 3), C.POINTER(C.c_int32 * 3), u32, u32, u32,
@@ -105,8 +105,9 @@ class OracleWorld:
         return int(lib().oracle_world_get_voxel(self._h, x, y, z))
 
     def cast(self, o: np.ndarray, d: np.ndarray, world_origin=(0, 0, 0), max_iters=128, begin=0, end=None,
-             with_pos=False, with_descents=False):
-        """Scalar cast; returns dict(t, rec[n,4] int32, pos[3,n] | None, descents | None) for rays [begin, end)."""
+             with_pos=False, with_descents=False, with_uv=False):
+        """Scalar cast; returns dict(t, rec[n,4] int32, pos[3,n] | None, descents | None, uv[2,n] | None)
+        for rays [begin, end)."""
         o = np.ascontiguousarray(o, dtype=np.float32)
         d = np.ascontiguousarray(d, dtype=np.float32)
         n = o.shape[1]
@@ -115,12 +116,14 @@ class OracleWorld:
         rec = np.zeros((n, 4), dtype=np.int32)
         pos = np.zeros((3, n), dtype=np.float32) if with_pos else None
         desc = np.zeros(n, dtype=np.uint16) if with_descents else None
+        uv = np.zeros((2, n), dtype=np.float32) if with_uv else None
         W = (C.c_int32 * 3)(*world_origin)
+        uvp = (P * 2)(uv[0].ctypes.data, uv[1].ctypes.data) if with_uv else None
         lib().oracle_cast(self._h, C.byref(_p3(o)), C.byref(_p3(d)), C.byref(W), begin, end, max_iters,
                           t.ctypes.data, rec.ctypes.data, C.byref(_p3(pos)) if with_pos else None,
-                          desc.ctypes.data if with_descents else None)
+                          desc.ctypes.data if with_descents else None, C.byref(uvp) if with_uv else None)
         return {"t": t[begin:end], "rec": rec[begin:end], "pos": pos[:, begin:end] if with_pos else None,
-                "descents": desc[begin:end] if with_descents else None}
+                "descents": desc[begin:end] if with_descents else None, "uv": uv[:, begin:end] if with_uv else None}
 
     def count(self, o, d, world_origin=(0, 0, 0), max_iters=128) -> dict:
         o = np.ascontiguousarray(o, dtype=np.float32)

@@ -168,8 +168,19 @@ static inline uint32_t normal_code(const float side[3], const float d[3], float 
     return code;
 }
 
+/* VHitResult::UV (CpuRenderer.cpp:217-220): fract(X ? pos.y : pos.x), fract(Z ? pos.y : pos.z)
+ * with fract(x) = x - floor(x) (SIMD_AVX2.h:121), X and Z the normal's tie masks. */
+static void hit_uv(const float side[3], float t, const float cur[3], float uv[2]) {
+    int X = side[0] == t, Y = side[1] == t;
+    int Z = !X && !Y;
+    float a = X ? cur[1] : cur[0], b = Z ? cur[1] : cur[2];
+    uv[0] = a - floorf(a);
+    uv[1] = b - floorf(b);
+}
+
 static void cast_one(const oracle_world* w, const float o[3], const float d[3], const int32_t W[3],
-                     uint32_t max_iters, float* t_out, oracle_hit* rec, float pos[3], uint32_t* desc_out) {
+                     uint32_t max_iters, float* t_out, oracle_hit* rec, float pos[3], uint32_t* desc_out,
+                     float uv[2]) {
     float inv[3], tstart[3], side[3] = {0.0f, 0.0f, 0.0f}, cur[3];
     for (int a = 0; a < 3; a++) {
         inv[a] = 1.0f / d[a];
@@ -204,29 +215,35 @@ static void cast_one(const oracle_world* w, const float o[3], const float d[3], 
             pos[1] = cur[1];
             pos[2] = cur[2];
         }
+        if (uv) hit_uv(side, t, cur, uv);
     } else {
         /* out of bounds after `iters` lookups, or the cap was exhausted (iters = max_iters) */
         *t_out = -1.0f;
         rec->x = rec->y = rec->z = 0;
         rec->info = (iters & 0xFFFFu) << 16;
         if (pos) pos[0] = pos[1] = pos[2] = 0.0f;
+        if (uv) uv[0] = uv[1] = 0.0f;
     }
     if (desc_out) *desc_out = descents;
 }
 
 void oracle_cast(const oracle_world* w, const float* const o[3], const float* const d[3],
                  const int32_t world_origin[3], uint32_t begin, uint32_t end, uint32_t max_iters,
-                 float* t, oracle_hit* rec, float* const pos[3], uint16_t* descents) {
+                 float* t, oracle_hit* rec, float* const pos[3], uint16_t* descents, float* const uv[2]) {
     for (uint32_t i = begin; i < end; i++) {
         float oi[3] = {o[0][i], o[1][i], o[2][i]};
         float di[3] = {d[0][i], d[1][i], d[2][i]};
-        float p[3];
+        float p[3], tc[2];
         uint32_t desc;
-        cast_one(w, oi, di, world_origin, max_iters, &t[i], &rec[i], pos ? p : NULL, &desc);
+        cast_one(w, oi, di, world_origin, max_iters, &t[i], &rec[i], pos ? p : NULL, &desc, uv ? tc : NULL);
         if (pos) {
             pos[0][i] = p[0];
             pos[1][i] = p[1];
             pos[2][i] = p[2];
+        }
+        if (uv) {
+            uv[0][i] = tc[0];
+            uv[1][i] = tc[1];
         }
         if (descents) descents[i] = (uint16_t)(desc > 0xFFFF ? 0xFFFF : desc);
     }
@@ -254,7 +271,7 @@ void oracle_count(const oracle_world* w, const float* const o[3], const float* c
         float t;
         oracle_hit r;
         uint32_t desc;
-        cast_one(w, oi, di, world_origin, max_iters, &t, &r, NULL, &desc);
+        cast_one(w, oi, di, world_origin, max_iters, &t, &r, NULL, &desc, NULL);
         out->rays++;
         out->iters += r.info >> 16;
         out->descents += desc;

@@ -234,3 +234,34 @@ def test_counting_variant_writes_the_same_hits(vrt, cuda, anchor):
         rec = plain.rec[:n].cpu().numpy()
         assert c[0] == n and c[3] == int(((rec[:, 3] & 0xFF) != 0).sum()), f"{anchor} {name}"
         assert c[1] == int(((rec[:, 3] >> 16) & 0xFFFF).sum()), f"{anchor} {name}"
+
+
+@pytest.mark.parametrize("W", [(0, 0, 0), (32, 64, 96), (5, -3, 7)])
+def test_uv_and_material_data_match_oracle(vrt, oracle_mod, cuda, W):
+    """The rest of VHitResult (CpuRenderer.cpp:210-221): UV bit-exact with the oracle's, and the
+    palette entry's low 32 bits of the hit material, for every world-origin specialization and
+    both schedules; other entry points reject the fields."""
+    import voxelrt_amd as V
+    from voxelrt_amd import _lib as L
+
+    vox = blob_world(128, seed=51, density=0.01)
+    world, ow = pair_from_dense(vrt, oracle_mod, vox)
+    rng = np.random.default_rng(52)
+    palette = rng.integers(0, 2**63, 256, dtype=np.int64).astype(np.uint64)
+    world.set_palette(palette)
+    o, d = random_rays(20000, 128, seed=53)
+    o = (o - np.array(W, np.float32)[:, None]).astype(np.float32)
+    ref = ow.cast(o, d, W, 128, with_pos=True, with_uv=True)
+    rb = V.RayBatch.from_numpy(o, d, W)
+    for sched, chunk in [(1, 0), (2, 64)]:
+        h = world.cast(rb, 128, with_pos=True, schedule=sched, chunk=chunk, with_uv=True, with_material=True)
+        torch.cuda.synchronize()
+        assert_hits_equal(h.t.cpu().numpy(), h.rec.cpu().numpy(), ref["t"], ref["rec"], f"W={W} sched={sched}")
+        assert np.array_equal(h.uv.cpu().numpy().view(np.uint32), ref["uv"].view(np.uint32))
+        mat = (ref["rec"][:, 3] & 0xFF).astype(np.int64)
+        want = np.where(mat != 0, (palette[mat] & np.uint64(0xFFFFFFFF)).astype(np.uint32), 0).astype(np.uint32)
+        assert np.array_equal(h.material_data.cpu().numpy().view(np.uint32), want)
+    hit = (ref["rec"][:, 3] & 0xFF) != 0
+    assert hit.mean() > 0.2 and np.all(ref["uv"][:, hit] >= 0) and np.all(ref["uv"][:, hit] <= 1)
+    with pytest.raises(V.VrtError, match="filled by vrt_cast"):
+        world.cast_shader(rb, out=V.HitBatch.empty(rb.n, with_uv=True))

@@ -292,3 +292,33 @@ def test_golden_fixture(oracle_mod):
     r = ow.cast(f["o"], f["d"], tuple(f["world_origin"]), int(f["max_iters"]))
     assert np.array_equal(r["rec"], f["rec"])
     assert np.array_equal(r["t"].view(np.uint32), f["t"].view(np.uint32))
+
+
+def test_uv_closed_form(oracle_mod):
+    """UV of a hit (CpuRenderer.cpp:217-220): on an x face (u, v) = fract(y, z), on a y face
+    fract(x, z), on a z face fract(x, y); at an x/y edge tie the x rule.  Nearly axis-aligned
+    rays hit a lone voxel's faces at known coordinates."""
+    vox = np.zeros((64, 64, 64), np.uint8)
+    vox[20, 20, 40] = 9  # (x, y, z) = (40, 20, 20)
+    ow = dense_world(oracle_mod, vox)
+    e = 2.0 ** -20
+    cases = [  # origin, direction, expected (u, v)
+        ((3.0, 20.25, 20.625), (1.0, e, e), (0.25, 0.625)),        # x face: (y, z)
+        ((40.375, 2.0, 20.75), (e, 1.0, e), (0.375, 0.75)),        # y face: (x, z)
+        ((40.125, 20.875, 60.0), (e, e, -1.0), (0.125, 0.875)),    # z face: (x, y)
+    ]
+    for o, d, (u, v) in cases:
+        r = ow.cast(np.array(o, np.float32).reshape(3, 1), np.array(d, np.float32).reshape(3, 1), (0, 0, 0), 128,
+                    with_pos=True, with_uv=True)
+        assert (r["rec"][0, 3] & 0xFF) == 9
+        assert abs(float(r["uv"][0, 0]) - u) < 1e-4 and abs(float(r["uv"][1, 0]) - v) < 1e-4, (o, r["uv"][:, 0])
+    # a miss has uv 0
+    r = ow.cast(np.array([[3.0], [50.5], [3.5]], np.float32), np.array([[1.0], [e], [e]], np.float32), (0, 0, 0),
+                128, with_uv=True)
+    assert (r["rec"][0, 3] & 0xFF) == 0 and r["uv"][0, 0] == 0 and r["uv"][1, 0] == 0
+    # all voxels solid: the ray starts in one, side distances tie at 0 on every axis: X and Y set,
+    # so u = fract(y), v = fract(z)
+    ow = dense_world(oracle_mod, np.full((64, 64, 64), 5, np.uint8))
+    r = ow.cast(np.array([[10.5], [20.25], [30.75]], np.float32), np.array([[0.3], [0.5], [0.81]], np.float32),
+                (0, 0, 0), 128, with_pos=True, with_uv=True)
+    assert float(r["uv"][0, 0]) == np.float32(0.25) and float(r["uv"][1, 0]) == np.float32(0.75)

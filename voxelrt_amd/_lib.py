@@ -78,7 +78,7 @@ class Hit(C.Structure):
 
 
 class Hits(C.Structure):
-    _fields_ = [("t", P), ("rec", P), ("pos", P)]
+    _fields_ = [("t", P), ("rec", P), ("pos", P), ("uv", P), ("material_data", P)]
 
 
 class CastCounters(C.Structure):

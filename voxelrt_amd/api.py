@@ -181,15 +181,19 @@ class HitBatch:
     t: torch.Tensor          # [n] float32, -1 on a miss
     rec: torch.Tensor        # [n, 4] int32: x, y, z, info
     pos: torch.Tensor | None  # [3, n] float32
+    uv: torch.Tensor | None = None             # [2, n] float32 (vrt_cast only)
+    material_data: torch.Tensor | None = None  # [n] int32: palette entry low 32 bits (vrt_cast only)
 
     @staticmethod
-    def empty(n: int, with_pos=False, device="cuda") -> "HitBatch":
+    def empty(n: int, with_pos=False, device="cuda", with_uv=False, with_material=False) -> "HitBatch":
         return HitBatch(torch.empty(n, dtype=torch.float32, device=device),
                         torch.empty((n, 4), dtype=torch.int32, device=device),
-                        torch.empty((3, n), dtype=torch.float32, device=device) if with_pos else None)
+                        torch.empty((3, n), dtype=torch.float32, device=device) if with_pos else None,
+                        torch.empty((2, n), dtype=torch.float32, device=device) if with_uv else None,
+                        torch.empty(n, dtype=torch.int32, device=device) if with_material else None)
 
     def to_c(self) -> L.Hits:
-        return L.Hits(self.t.data_ptr(), self.rec.data_ptr(), self.pos.data_ptr() if self.pos is not None else 0)
+        return L.Hits(self.t.data_ptr(), self.rec.data_ptr(), _ptr(self.pos), _ptr(self.uv), _ptr(self.material_data))
 
     # decoding of vrt_hit.info
     def material(self) -> torch.Tensor:
@@ -332,11 +336,11 @@ class World:
     def cast(self, rays: RayBatch, max_iters: int = 128, with_pos: bool = False, schedule: int = 0,
              chunk: int = 0, waves_per_cu: int = 0, counters: torch.Tensor | None = None,
              out: HitBatch | None = None, count: torch.Tensor | None = None, refill: int = 0,
-             order: int = 0, stream=None) -> HitBatch:
+             order: int = 0, with_uv: bool = False, with_material: bool = False, stream=None) -> HitBatch:
         """Trace rays (RayCast, CpuRenderer.cpp:172-224).  count: optional device int32 holding the
         number of rays to trace (rays.n is then the capacity); order: the grid's block order
         (VRT_ORDER_FORWARD / VRT_ORDER_REVERSE, same results)."""
-        hits = out if out is not None else HitBatch.empty(rays.n, with_pos, rays.o.device)
+        hits = out if out is not None else HitBatch.empty(rays.n, with_pos, rays.o.device, with_uv, with_material)
         r = rays.to_c()
         h = hits.to_c()
         opts = L.CastOpts(schedule, refill, chunk, waves_per_cu, counters.data_ptr() if counters is not None else 0,

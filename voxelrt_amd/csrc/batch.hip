@@ -304,6 +304,8 @@ static int create_plan(const vrt_world* world, const vrt_batch* b, const vrt_bat
     if (b->nsegments && !b->segments) return fail(VRT_ERR_INVALID_ARGUMENT, "batch: null segment table");
     if (b->ncameras && !b->cameras) return fail(VRT_ERR_INVALID_ARGUMENT, "batch: null camera table");
     if (((uintptr_t)b->hits.rec & 15) != 0) return fail(VRT_ERR_INVALID_ARGUMENT, "batch: hit records must be 16-byte aligned");
+    if (b->hits.uv || b->hits.material_data)
+        return fail(VRT_ERR_INVALID_ARGUMENT, "batch: uv and material_data are filled by vrt_cast only");
     const uint32_t per_block = 64 * tun->rays_per_lane;
     std::vector<BatchSegDev> segs(b->nsegments);
     uint64_t blocks = 0, rays = 0;

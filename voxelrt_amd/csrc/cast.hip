@@ -50,11 +50,22 @@ __device__ __forceinline__ void flush_counters(vrt_cast_counters* ctr, unsigned 
 __device__ __forceinline__ void store_result(const HitsOut& h, uint32_t i, const RayState& r, float t, int4 rec) {
     h.t[i] = t;
     h.rec[i] = rec;
+    const bool hit = (rec.w & 0xFF) != 0;
     if (h.pos_x) {
-        h.pos_x[i] = rec.w & 0xFF ? r.cur[0] : 0.0f;
-        h.pos_y[i] = rec.w & 0xFF ? r.cur[1] : 0.0f;
-        h.pos_z[i] = rec.w & 0xFF ? r.cur[2] : 0.0f;
+        h.pos_x[i] = hit ? r.cur[0] : 0.0f;
+        h.pos_y[i] = hit ? r.cur[1] : 0.0f;
+        h.pos_z[i] = hit ? r.cur[2] : 0.0f;
     }
+    if (h.uv_u) {
+        // UV of CpuRenderer.cpp:217-220: fract(X ? pos.y : pos.x), fract(Z ? pos.y : pos.z), the
+        // tie masks X and Z being the nonzero normal fields of the record
+        bool X = ((uint32_t)rec.w >> 8) & 3u, Z = ((uint32_t)rec.w >> 12) & 3u;
+        float a = X ? r.cur[1] : r.cur[0], b = Z ? r.cur[1] : r.cur[2];
+        h.uv_u[i] = hit ? a - floorf(a) : 0.0f;
+        h.uv_v[i] = hit ? b - floorf(b) : 0.0f;
+    }
+    // the palette entry's low 32 bits (GetVoxelMaterial, CpuRenderer.cpp:120-132)
+    if (h.material_data) h.material_data[i] = hit ? (uint32_t)h.palette[rec.w & 0xFF] : 0u;
 }
 
 // Traces one ray to completion.
@@ -308,8 +319,10 @@ int launch_cast(const vrt_world* world, const RaysIn& r, const HitsOut& h, uint3
     return VRT_OK;
 }
 
-int check_rays_hits(const vrt_rays* rays, const vrt_hits* hits, RaysIn& r, HitsOut& h) {
+int check_rays_hits(const vrt_rays* rays, const vrt_hits* hits, RaysIn& r, HitsOut& h, bool extras) {
     if (!rays || !hits) return fail(VRT_ERR_INVALID_ARGUMENT, "null rays or hits");
+    if (!extras && (hits->uv || hits->material_data))
+        return fail(VRT_ERR_INVALID_ARGUMENT, "uv and material_data are filled by vrt_cast / vrt_cast_ex only");
     if (rays->n && (!rays->ox || !rays->oy || !rays->oz || !rays->dx || !rays->dy || !rays->dz))
         return fail(VRT_ERR_INVALID_ARGUMENT, "null ray buffer");
     if (rays->n && (!hits->t || !hits->rec)) return fail(VRT_ERR_INVALID_ARGUMENT, "null hit buffer");
@@ -328,6 +341,10 @@ int check_rays_hits(const vrt_rays* rays, const vrt_hits* hits, RaysIn& r, HitsO
     } else {
         h.pos_x = h.pos_y = h.pos_z = nullptr;
     }
+    h.uv_u = hits->uv;
+    h.uv_v = hits->uv ? hits->uv + rays->n : nullptr;
+    h.material_data = hits->material_data;
+    h.palette = nullptr;
     return VRT_OK;
 }
 
@@ -430,8 +447,9 @@ int vrt_cast_ex(const vrt_world* world, const vrt_rays* rays, uint32_t max_iters
         if (fault_injected("vrt_cast")) return fail(VRT_ERR_HIP, "vrt_cast: injected HIP fault");
         RaysIn r;
         HitsOut h;
-        int rc = check_rays_hits(rays, hits, r, h);
+        int rc = check_rays_hits(rays, hits, r, h, true);
         if (rc != VRT_OK) return rc;
+        h.palette = world->d_palette;
         vrt_cast_opts o;
         std::memset(&o, 0, sizeof(o));
         if (opts) o = *opts;

@@ -285,7 +285,10 @@ extern "C" int vrt_cast_frame(const vrt_world* world, const vrt_camera* cam, uin
         a.n = (uint32_t)n64;
         a.max_iters = max_iters;
         a.seed = seed;
+        if (primary->uv || primary->material_data || bounce->uv || bounce->material_data)
+            return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_cast_frame: uv and material_data are filled by vrt_cast only");
         auto hits_view = [&](const vrt_hits* hh, HitsOut& out) {
+            out = HitsOut{};
             out.t = hh->t;
             out.rec = (int4*)hh->rec;
             out.pos_x = hh->pos;

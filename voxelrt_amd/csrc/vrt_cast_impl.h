@@ -31,6 +31,10 @@ struct HitsOut {
     float* __restrict__ pos_x;  // all three null when positions are not requested
     float* __restrict__ pos_y;
     float* __restrict__ pos_z;
+    float* __restrict__ uv_u;   // both null when texture coordinates are not requested
+    float* __restrict__ uv_v;
+    uint32_t* __restrict__ material_data;  // null when not requested
+    const uint64_t* __restrict__ palette;  // the world's palette (material_data)
 };
 
 // ---------------------------------------------------------------- work queues
@@ -63,8 +67,9 @@ __device__ __forceinline__ uint32_t claim_chunk(const ChunkQueue& cq, uint32_t h
 }
 
 
-// Validates the ABI structs and converts them to kernel views.
-int check_rays_hits(const vrt_rays* rays, const vrt_hits* hits, RaysIn& r, HitsOut& h);
+// Validates the ABI structs and converts them to kernel views; with extras = false the
+// optional uv and material_data outputs are rejected (entry points that do not fill them).
+int check_rays_hits(const vrt_rays* rays, const vrt_hits* hits, RaysIn& r, HitsOut& h, bool extras = false);
 // Launches a cast with the given options on stream s.
 int launch_cast(const vrt_world* world, const RaysIn& r, const HitsOut& h, uint32_t max_iters,
                 const vrt_cast_opts& o, hipStream_t s);
