@@ -236,6 +236,26 @@ def test_counting_variant_writes_the_same_hits(vrt, cuda, anchor):
         assert c[1] == int(((rec[:, 3] >> 16) & 0xFFFF).sum()), f"{anchor} {name}"
 
 
+@pytest.mark.parametrize("W", [(0, 0, 0), (32, -64, 96), (-7, 3, 11)])
+def test_counting_variant_non_cubic_world(vrt, oracle_mod, cuda, W):
+    """The counting instantiation on a non-cubic world (the two-compare bounds test after the
+    loop) at each world-origin specialization: records bit-exact with the oracle, counter
+    totals equal to the oracle's."""
+    vox = blob_world(128, seed=41, density=0.01, ysize=64)
+    world, ow = pair_from_dense(vrt, oracle_mod, vox)
+    for inside in (True, False):
+        o, d = random_rays(20000, 128, seed=42, ysize=64, inside=inside)
+        o = (o - np.array(W, np.float32)[:, None]).astype(np.float32)
+        ref = ow.cast(o, d, W, 128)
+        cref = ow.count(o, d, W, 128)
+        for sched in (1, -1):
+            ctr = vrt.new_counters()
+            t, rec, _ = gpu_cast(world, o, d, W, 128, sched, counters=ctr)
+            assert_hits_equal(t, rec, ref["t"], ref["rec"], f"W={W} inside={inside} sched={sched}")
+            c = ctr.cpu().numpy()
+            assert (c[0], c[1], c[2], c[3]) == (o.shape[1], cref["iters"], cref["descents"], cref["hits"])
+
+
 @pytest.mark.parametrize("W", [(0, 0, 0), (32, 64, 96), (5, -3, 7)])
 def test_uv_and_material_data_match_oracle(vrt, oracle_mod, cuda, W):
     """The rest of VHitResult (CpuRenderer.cpp:210-221): UV bit-exact with the oracle's, and the

@@ -167,4 +167,11 @@ def test_oversized_launches_are_rejected_before_any_device_work(vrt):
     assert lib.vrt_cast_shader(fake_world, C.byref(rays), C.byref(opts), C.byref(hits), None) == \
         L.VRT_ERR_INVALID_ARGUMENT
     assert b"too many rays" in lib.vrt_last_error()
+    # FP64 picking cast: its loop runs every step up to the cap, so the cap is 1..65535
+    o3, d3 = (C.c_double * 3)(0.5, 0.5, 0.5), (C.c_double * 3)(1.0, 0.0, 0.0)
+    hit = L.HitF64()
+    for cap in (0, 65536, 0xFFFFFFFF):
+        assert lib.vrt_cast_one_f64(fake_world, C.byref(o3), C.byref(d3), cap, C.byref(hit)) == \
+            L.VRT_ERR_INVALID_ARGUMENT
+        assert b"1..65535" in lib.vrt_last_error()
 

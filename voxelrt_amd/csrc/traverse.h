@@ -280,10 +280,13 @@ __device__ __forceinline__ int ray_step(const WorldView& w, RayState& r, int32_t
 // Brick slot of voxel v, which lies in an allocated brick (a hit voxel): the
 // lookup of a descending step, repeated once after the loop so the loop does
 // not carry the slot.
+// The record is read by the range-checked structured load: an index past the records
+// reads 0 (slot = 0), so no v can make this lookup leave the world's buffers.
 __device__ __forceinline__ uint32_t brick_slot_at(const WorldView& w, const int32_t v[3]) {
     uint32_t ux = (uint32_t)v[0], uy = (uint32_t)v[1], uz = (uint32_t)v[2];
     uint32_t si = (ux >> 5) | (uz >> 5) << w.shift_xz | (uy >> 5) << (2 * w.shift_xz);
-    uint4 sec = w.sec[si];
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    u32x4 sec = (u32x4)vrt_struct_buffer_load_v4i32(sector_records_rsrc(w), (int32_t)si, 0, 0, 0);
     uint64_t m = (uint64_t)sec.x | (uint64_t)sec.y << 32;
     uint32_t bi = __builtin_amdgcn_ubfe(ux, 3, 2) | __builtin_amdgcn_ubfe(uz, 3, 2) << 2 |
                   __builtin_amdgcn_ubfe(uy, 3, 2) << 4;
@@ -334,11 +337,8 @@ __device__ __forceinline__ bool ray_trace(const WorldView& w, RayState& r, uint3
         r.iters += h ? 1u : 0u;
         return h;
     }
-    // a hit is a ray stopped inside the world before the cap; the count and v pass through
-    // an empty asm statement, without which the counting kernel's instance of this test
-    // was miscompiled (its hits and counts went wrong; the plain instance was right)
+    // a hit is a ray stopped inside the world before the cap
     uint32_t steps = r.iters;
-    asm volatile("" : "+v"(steps), "+v"(v[0]), "+v"(v[1]), "+v"(v[2]));
     bool h = steps < max_iters && (CUBIC ? (uint32_t)(v[0] | v[1] | v[2]) < w.size_xz
                                          : ((uint32_t)(v[0] | v[2]) < w.size_xz && (uint32_t)v[1] < w.size_y));
     if (h) slot = brick_slot_at(w, v);
