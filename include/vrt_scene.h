@@ -75,6 +75,37 @@ int vrt_scene_get(const vrt_scene* scene, vrt_sector_stream* out);
 uint8_t vrt_scene_voxel(const vrt_scene* scene, int32_t x, int32_t y, int32_t z);
 
 /*
+ * Editing the host world: the reference's VoxelMap (src/VoxelRT/VoxelMap.h:180-211).
+ * The scene keeps a dirty set, VoxelMap::DirtyLocs (VoxelMap.h:185): sector index ->
+ * mask of the bricks changed since the last vrt_world_sync_scene.
+ *
+ * vrt_scene_voxel is VoxelMap::Get (VoxelMap.h:191-194): 0 outside the scene or in an
+ * absent brick.
+ *
+ * vrt_scene_set_voxel is VoxelMap::Set (VoxelMap.h:195-201): writes one voxel,
+ * allocating its brick (zero-filled) when absent, also for value 0, and marks the
+ * brick dirty (GetBrick(create, markAsDirty), VoxelMap.cpp:98-123).  A position
+ * outside the scene's extents is ignored and returns VRT_OK, as Set ignores positions
+ * outside the reference's sector grid.  O(1) per call: edited sectors are held aside
+ * and folded into the sector stream by the next call that reads the stream
+ * (vrt_scene_get, vrt_scene_save_cvox, vrt_world_from_scene).
+ */
+int vrt_scene_set_voxel(vrt_scene* scene, int32_t x, int32_t y, int32_t z, uint8_t value);
+
+/* VoxelMap::MarkAllDirty (VoxelMap.h:207-211): every sector of the scene becomes dirty
+ * with its whole allocation mask. */
+int vrt_scene_mark_all_dirty(vrt_scene* scene);
+
+/* Deletes the scene's bricks that hold no solid voxel and marks them dirty; a sector left
+ * without bricks leaves the scene (the garbage collection of VoxelMap::RegionDispatchSIMD,
+ * VoxelMap.h:236-263, over the whole scene; Sector::DeleteEmptyBricks, VoxelMap.cpp:67-79).
+ * *deleted (may be NULL) receives the number of bricks deleted. */
+int vrt_scene_delete_empty_bricks(vrt_scene* scene, uint64_t* deleted);
+
+/* Number of sectors in the scene's dirty set. */
+uint32_t vrt_scene_dirty_sectors(const vrt_scene* scene);
+
+/*
  * The reference's world file, "cvox 0004" (VoxelMap::Deserialize / Serialize,
  * src/VoxelRT/VoxelMap.cpp:207-274; zstd blobs of BinaryIO.cpp:8-60): a palette
  * of 256 Material records and the sectors of a VoxelMap, indexed by
@@ -106,6 +137,17 @@ void vrt_scene_destroy(vrt_scene* scene);
 
 /* Convenience: vrt_world_create + vrt_world_sync of the whole scene. */
 int vrt_world_from_scene(const vrt_scene* scene, void* stream, vrt_world** out);
+
+/*
+ * GpuVoxelStorage::SyncBuffers(VoxelMap&) (src/VoxelRT/GpuRenderer.cpp:45-167): consumes
+ * the scene's dirty set.  Every dirty sector is synced with the scene's current allocation
+ * mask (0 for a sector the scene no longer holds) and its dirty bricks that are still
+ * allocated (dirtyMask &= allocMask, GpuRenderer.cpp:62), through vrt_world_sync_bricks;
+ * then the scene's palette is uploaded, on every call as SyncBuffers does
+ * (GpuRenderer.cpp:100-102).  The world's extents must equal the scene's (the same sector
+ * indices).  On success the dirty set is cleared; on failure it is kept.
+ */
+int vrt_world_sync_scene(vrt_world* world, vrt_scene* scene, void* stream);
 
 /*
  * Camera set-up: eye at `pos` (absolute, double), yaw/pitch in degrees

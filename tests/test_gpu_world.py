@@ -368,6 +368,56 @@ def test_brick_sync_uploads_only_dirty_bricks(vrt, oracle_mod, cuda):
     check("rejected")
 
 
+def test_scene_edits_sync_through_the_dirty_set(vrt, oracle_mod, cuda):
+    """The reference's editing loop: VoxelMap::Set marks DirtyLocs, SyncBuffers(VoxelMap&)
+    consumes it (vrt_scene_set_voxel, vrt_world_sync_scene).  Each round of edits uploads
+    exactly the dirty bricks still allocated, clears the dirty set, and leaves the world's casts
+    equal to the oracle's on the edited dense grid."""
+    import voxelrt_amd as V
+    from test_oracle import dense_world
+
+    rng = np.random.default_rng(23)
+    vox = blob_world(128, seed=43, density=0.01)
+    scene = V.Scene.from_dense(vox)
+    world = V.World.from_scene(scene)
+    o, d = random_rays(20000, 128, seed=24)
+    for rnd in range(4):
+        dirty = {}
+        for _ in range(300 if rnd < 3 else 0):
+            x, y, z = (int(c) for c in rng.integers(0, 128, 3))
+            v = int(rng.integers(0, 256)) if rng.random() < 0.8 else 0
+            scene.set_voxel(x, y, z, v)
+            vox[y, z, x] = v
+            s, b = brick_of(x, y, z)
+            dirty[s] = dirty.get(s, 0) | 1 << b
+        if rnd == 3:
+            # carve a whole sector empty, then collect the empty bricks
+            for y in range(64, 96):
+                for z in range(0, 32):
+                    for x in range(32, 64):
+                        if vox[y, z, x]:
+                            scene.set_voxel(x, y, z, 0)
+                            vox[y, z, x] = 0
+                            s, b = brick_of(x, y, z)
+                            dirty[s] = dirty.get(s, 0) | 1 << b
+            scene.delete_empty_bricks()
+        assert scene.dirty_sectors >= len(dirty)
+        # every dirty brick that is still allocated goes up (Set allocates even for 0)
+        st = scene.stream()
+        present = dict(zip(st["sector_idx"].tolist(), st["alloc_mask"].tolist()))
+        alloc = {s: int(present.get(s, 0)) for s in dirty}
+        n_up = sum(bin(dm & alloc[s]).count("1") for s, dm in dirty.items())
+        i0 = world.info()
+        world.sync_scene(scene)
+        assert scene.dirty_sectors == 0
+        i1 = world.info()
+        assert i1.upload_brick_bytes - i0.upload_brick_bytes == 512 * n_up, f"round {rnd}"
+        ref = dense_world(oracle_mod, vox).cast(o, d, (0, 0, 0), 128)
+        t, rec = cast_np(world, o, d)
+        assert_hits_equal(t, rec, ref["t"], ref["rec"], f"round {rnd}")
+        assert world.info().num_bricks == scene.stream()["nbricks"]
+
+
 def test_sync_staging_failure_leaves_world_unchanged(vrt, cuda):
     """The staging buffers are allocated by a world's first sync, before any device work: a failure
     there returns an error and the world stays as it was (empty), usable by the next sync."""

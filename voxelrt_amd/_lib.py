@@ -196,11 +196,16 @@ _SIGNATURES = {
     "vrt_scene_from_dense": (C.c_int, [u32, u32, P, C.POINTER(P)]),
     "vrt_scene_get": (C.c_int, [P, C.POINTER(SectorStream)]),
     "vrt_scene_voxel": (C.c_uint8, [P, i32, i32, i32]),
+    "vrt_scene_set_voxel": (C.c_int, [P, i32, i32, i32, C.c_uint8]),
+    "vrt_scene_mark_all_dirty": (C.c_int, [P]),
+    "vrt_scene_delete_empty_bricks": (C.c_int, [P, C.POINTER(C.c_uint64)]),
+    "vrt_scene_dirty_sectors": (u32, [P]),
     "vrt_scene_destroy": (None, [P]),
     "vrt_scene_load_cvox": (C.c_int, [C.c_char_p, u32, u32, C.POINTER(i32 * 3), C.POINTER(P), C.POINTER(CvoxInfo)]),
     "vrt_scene_save_cvox": (C.c_int, [P, C.POINTER(i32 * 3), P, C.c_char_p]),
     "vrt_scene_get_palette": (C.c_int, [P, P]),
     "vrt_world_from_scene": (C.c_int, [P, P, C.POINTER(P)]),
+    "vrt_world_sync_scene": (C.c_int, [P, P, P]),
     "vrt_camera_look": (C.c_int, [C.POINTER(C.c_double * 3), C.c_double, C.c_double, C.c_double, u32, u32, u32, u32,
                                   C.POINTER(Camera)]),
     "vrt_camera_rays_host": (C.c_int, [C.POINTER(Camera), C.POINTER(P * 6)]),

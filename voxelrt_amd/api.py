@@ -123,7 +123,28 @@ class Scene:
         }
 
     def voxel(self, x: int, y: int, z: int) -> int:
+        """VoxelMap::Get: 0 outside the scene or in an absent brick."""
         return int(L.lib().vrt_scene_voxel(self._h, x, y, z))
+
+    def set_voxel(self, x: int, y: int, z: int, value: int) -> None:
+        """VoxelMap::Set: writes one voxel, allocating its brick, and marks the brick dirty;
+        positions outside the scene are ignored."""
+        L.check(L.lib().vrt_scene_set_voxel(self._h, x, y, z, value))
+
+    def mark_all_dirty(self) -> None:
+        """VoxelMap::MarkAllDirty: every sector dirty with its whole allocation mask."""
+        L.check(L.lib().vrt_scene_mark_all_dirty(self._h))
+
+    def delete_empty_bricks(self) -> int:
+        """Deletes bricks without a solid voxel (marking them dirty); returns how many."""
+        n = C.c_uint64()
+        L.check(L.lib().vrt_scene_delete_empty_bricks(self._h, C.byref(n)))
+        return int(n.value)
+
+    @property
+    def dirty_sectors(self) -> int:
+        """Number of sectors in the dirty set (VoxelMap::DirtyLocs)."""
+        return int(L.lib().vrt_scene_dirty_sectors(self._h))
 
     def close(self) -> None:
         if self._h and self._h.value:
@@ -317,6 +338,10 @@ class World:
         br = np.ascontiguousarray(bricks, dtype=np.uint8)
         L.check(L.lib().vrt_world_sync_bricks(self._h, len(si), si.ctypes.data, am.ctypes.data, dm.ctypes.data,
                                               br.ctypes.data if br.size else 0, _stream_handle(stream)))
+
+    def sync_scene(self, scene: Scene, stream=None) -> None:
+        """SyncBuffers(VoxelMap&): consumes the scene's dirty set (vrt_world_sync_scene)."""
+        L.check(L.lib().vrt_world_sync_scene(self._h, scene._h, _stream_handle(stream)))
 
     def set_palette(self, palette: np.ndarray, stream=None) -> None:
         p = np.ascontiguousarray(palette, dtype=np.uint64)

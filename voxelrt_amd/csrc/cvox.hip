@@ -320,6 +320,7 @@ int vrt_scene_save_cvox(const vrt_scene* sc, const int32_t sector_offset[3], con
                         const char* path) {
     return guarded([&]() -> int {
         if (!sc || !path) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_scene_save_cvox: null argument");
+        scene_pack_edits(const_cast<vrt_scene*>(sc));
         const Zstd& z = zstd();
         if (!z.error.empty()) return fail(VRT_ERR_STATE, "vrt_scene_save_cvox: " + z.error);
         int32_t off[3] = {0, 0, 0};

@@ -12,7 +12,9 @@
 #include <bit>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "vrt_internal.h"
@@ -185,7 +187,94 @@ vrt_scene* new_scene(uint32_t lxz, uint32_t ly) {
     return sc;
 }
 
+// Brick index of voxel (x, y, z) within its sector, and the voxel's index within the brick.
+inline uint32_t brick_of(uint32_t x, uint32_t y, uint32_t z) { return ((x >> 3) & 3u) | ((z >> 3) & 3u) << 2 | ((y >> 3) & 3u) << 4; }
+inline uint32_t voxel_of(uint32_t x, uint32_t y, uint32_t z) { return (x & 7u) | (z & 7u) << 3 | (y & 7u) << 6; }
+
+uint32_t sector_of(const vrt_scene* sc, uint32_t x, uint32_t y, uint32_t z) {
+    return (x >> 5) | (z >> 5) << sc->shift_xz | (y >> 5) << (2 * sc->shift_xz);
+}
+
+// The sector's mask in the scene (the edit overlay first, then the packed stream).
+uint64_t scene_mask(const vrt_scene* sc, uint32_t si) {
+    if (auto it = sc->edits.find(si); it != sc->edits.end()) return it->second.mask;
+    int32_t k = sc->sector_lookup[si];
+    return k < 0 ? 0ull : sc->alloc_mask[k];
+}
+
+// Brick b of sector si (allocated), from the overlay or the packed stream.
+const uint8_t* scene_brick(const vrt_scene* sc, uint32_t si, uint32_t b) {
+    if (auto it = sc->edits.find(si); it != sc->edits.end()) return it->second.bricks.data() + (size_t)b * 512;
+    int32_t k = sc->sector_lookup[si];
+    uint64_t m = sc->alloc_mask[k];
+    return sc->bricks.data() + (sc->brick_offset[k] + (uint64_t)std::popcount(m & ((1ull << b) - 1))) * 512;
+}
+
+// The overlay entry of sector si, copied out of the packed stream on first use.
+vrt_edit_sector& edit_sector(vrt_scene* sc, uint32_t si) {
+    auto [it, fresh] = sc->edits.try_emplace(si);
+    vrt_edit_sector& e = it->second;
+    if (fresh) {
+        e.bricks.assign(64 * 512, 0);
+        int32_t k = sc->sector_lookup[si];
+        if (k >= 0) {
+            e.mask = sc->alloc_mask[k];
+            const uint8_t* src = sc->bricks.data() + sc->brick_offset[k] * 512;
+            for (uint64_t m = e.mask; m; m &= m - 1, src += 512)
+                std::memcpy(e.bricks.data() + (size_t)std::countr_zero(m) * 512, src, 512);
+        }
+    }
+    return e;
+}
+
 }  // namespace
+
+void scene_pack_edits(vrt_scene* sc) {
+    if (sc->edits.empty()) return;
+    std::vector<uint32_t> idx;
+    std::vector<uint64_t> mask, offset;
+    std::vector<uint8_t> bricks;
+    bricks.reserve(sc->bricks.size());
+    std::vector<uint32_t> edited;
+    for (auto& [si, e] : sc->edits) edited.push_back(si);
+    std::sort(edited.begin(), edited.end());
+    // merge the packed sectors with the edited ones, in increasing sector index
+    size_t a = 0, b = 0;
+    auto emit = [&](uint32_t si, uint64_t m, const uint8_t* packed, const vrt_edit_sector* e) {
+        if (!m) return;
+        idx.push_back(si);
+        mask.push_back(m);
+        offset.push_back(bricks.size() / 512);
+        if (e) {
+            for (uint64_t r = m; r; r &= r - 1) {
+                const uint8_t* src = e->bricks.data() + (size_t)std::countr_zero(r) * 512;
+                bricks.insert(bricks.end(), src, src + 512);
+            }
+        } else {
+            bricks.insert(bricks.end(), packed, packed + (size_t)std::popcount(m) * 512);
+        }
+    };
+    while (a < sc->sector_idx.size() || b < edited.size()) {
+        uint32_t sa = a < sc->sector_idx.size() ? sc->sector_idx[a] : UINT32_MAX;
+        uint32_t sb = b < edited.size() ? edited[b] : UINT32_MAX;
+        if (sb <= sa) {
+            const vrt_edit_sector& e = sc->edits.at(sb);
+            emit(sb, e.mask, nullptr, &e);
+            b++;
+            if (sa == sb) a++;
+        } else {
+            emit(sa, sc->alloc_mask[a], sc->bricks.data() + sc->brick_offset[a] * 512, nullptr);
+            a++;
+        }
+    }
+    for (int32_t& k : sc->sector_lookup) k = -1;
+    for (size_t k = 0; k < idx.size(); k++) sc->sector_lookup[idx[k]] = (int32_t)k;
+    sc->sector_idx.swap(idx);
+    sc->alloc_mask.swap(mask);
+    sc->brick_offset.swap(offset);
+    sc->bricks.swap(bricks);
+    sc->edits.clear();
+}
 
 extern "C" {
 
@@ -339,6 +428,7 @@ int vrt_scene_from_dense(uint32_t lxz, uint32_t ly, const uint8_t* voxels, vrt_s
 int vrt_scene_get(const vrt_scene* sc, vrt_sector_stream* out) {
     return guarded([&]() -> int {
         if (!sc || !out) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_scene_get: null argument");
+        scene_pack_edits(const_cast<vrt_scene*>(sc));
         out->log2_size_xz = sc->log2_xz;
         out->log2_size_y = sc->log2_y;
         out->nsec = (uint32_t)sc->sector_idx.size();
@@ -354,20 +444,124 @@ int vrt_scene_get(const vrt_scene* sc, vrt_sector_stream* out) {
 uint8_t vrt_scene_voxel(const vrt_scene* sc, int32_t x, int32_t y, int32_t z) {
     if (!sc) return 0;
     if (!((uint32_t)(x | z) < (1u << sc->log2_xz) && (uint32_t)y < (1u << sc->log2_y))) return 0;
-    uint32_t si = (uint32_t)(x >> 5) | (uint32_t)(z >> 5) << sc->shift_xz | (uint32_t)(y >> 5) << (2 * sc->shift_xz);
-    int32_t k = sc->sector_lookup[si];
-    if (k < 0) return 0;
-    uint64_t m = sc->alloc_mask[k];
-    uint32_t b = ((uint32_t)(x >> 3) & 3u) | ((uint32_t)(z >> 3) & 3u) << 2 | ((uint32_t)(y >> 3) & 3u) << 4;
-    if (!((m >> b) & 1)) return 0;
-    uint64_t slot = sc->brick_offset[k] + (uint64_t)std::popcount(m & ((1ull << b) - 1));
-    return sc->bricks[slot * 512 + (((uint32_t)x & 7u) | ((uint32_t)z & 7u) << 3 | ((uint32_t)y & 7u) << 6)];
+    uint32_t ux = (uint32_t)x, uy = (uint32_t)y, uz = (uint32_t)z;
+    uint32_t si = sector_of(sc, ux, uy, uz), b = brick_of(ux, uy, uz);
+    if (!((scene_mask(sc, si) >> b) & 1)) return 0;
+    return scene_brick(sc, si, b)[voxel_of(ux, uy, uz)];
+}
+
+int vrt_scene_set_voxel(vrt_scene* sc, int32_t x, int32_t y, int32_t z, uint8_t value) {
+    return guarded([&]() -> int {
+        if (!sc) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_scene_set_voxel: null scene");
+        // outside the scene: ignored, as VoxelMap::Set ignores positions outside its sector grid
+        if (!((uint32_t)(x | z) < (1u << sc->log2_xz) && (uint32_t)y < (1u << sc->log2_y))) return VRT_OK;
+        uint32_t ux = (uint32_t)x, uy = (uint32_t)y, uz = (uint32_t)z;
+        uint32_t si = sector_of(sc, ux, uy, uz), b = brick_of(ux, uy, uz);
+        vrt_edit_sector& e = edit_sector(sc, si);
+        e.mask |= 1ull << b;  // GetBrick(create = true): a new brick starts empty
+        uint8_t& v = e.bricks[(size_t)b * 512 + voxel_of(ux, uy, uz)];
+        sc->solid += (value != 0) - (v != 0);
+        v = value;
+        sc->dirty[si] |= 1ull << b;  // markAsDirty (VoxelMap.cpp:118-121)
+        return VRT_OK;
+    });
+}
+
+int vrt_scene_mark_all_dirty(vrt_scene* sc) {
+    return guarded([&]() -> int {
+        if (!sc) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_scene_mark_all_dirty: null scene");
+        for (size_t k = 0; k < sc->sector_idx.size(); k++)
+            if (!sc->edits.count(sc->sector_idx[k])) sc->dirty[sc->sector_idx[k]] = sc->alloc_mask[k];
+        for (auto& [si, e] : sc->edits)
+            if (e.mask) sc->dirty[si] = e.mask;
+        return VRT_OK;
+    });
+}
+
+int vrt_scene_delete_empty_bricks(vrt_scene* sc, uint64_t* deleted) {
+    return guarded([&]() -> int {
+        if (!sc) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_scene_delete_empty_bricks: null scene");
+        uint64_t count = 0;
+        auto empty_brick = [](const uint8_t* p) {
+            for (uint32_t i = 0; i < 512; i++)
+                if (p[i]) return false;
+            return true;
+        };
+        // the packed sectors that hold an empty brick move into the overlay first
+        std::vector<uint32_t> todo;
+        for (size_t k = 0; k < sc->sector_idx.size(); k++) {
+            uint32_t si = sc->sector_idx[k];
+            if (sc->edits.count(si)) continue;
+            const uint8_t* src = sc->bricks.data() + sc->brick_offset[k] * 512;
+            for (uint64_t m = sc->alloc_mask[k]; m; m &= m - 1, src += 512)
+                if (empty_brick(src)) {
+                    todo.push_back(si);
+                    break;
+                }
+        }
+        for (uint32_t si : todo) edit_sector(sc, si);
+        for (auto& [si, e] : sc->edits) {
+            uint64_t gone = 0;
+            for (uint64_t m = e.mask; m; m &= m - 1) {
+                uint32_t b = (uint32_t)std::countr_zero(m);
+                if (empty_brick(e.bricks.data() + (size_t)b * 512)) gone |= 1ull << b;
+            }
+            if (!gone) continue;
+            e.mask &= ~gone;
+            sc->dirty[si] |= gone;  // the reference marks the emptied bricks dirty before collecting them
+            count += (uint64_t)std::popcount(gone);
+        }
+        if (deleted) *deleted = count;
+        return VRT_OK;
+    });
+}
+
+uint32_t vrt_scene_dirty_sectors(const vrt_scene* sc) { return sc ? (uint32_t)sc->dirty.size() : 0u; }
+
+int vrt_world_sync_scene(vrt_world* world, vrt_scene* sc, void* stream) {
+    return guarded([&]() -> int {
+        if (!world || !sc) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_world_sync_scene: null argument");
+        vrt_world_info info;
+        int rc = vrt_world_get_info(world, &info);
+        if (rc != VRT_OK) return rc;
+        if (info.size_xz != (1u << sc->log2_xz) || info.size_y != (1u << sc->log2_y))
+            return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_world_sync_scene: the world's extents differ from the scene's");
+        // SyncBuffers (GpuRenderer.cpp:51-79): each dirty sector's mask now (0 for a sector the
+        // scene no longer holds) and its dirty bricks that are still allocated
+        std::vector<uint32_t> idx;
+        std::vector<uint64_t> alloc, dirty;
+        std::vector<uint8_t> bytes;
+        for (auto& [si, d] : sc->dirty) {
+            uint64_t m = scene_mask(sc, si), dm = d & m;
+            idx.push_back(si);
+            alloc.push_back(m);
+            dirty.push_back(dm);
+            for (uint64_t r = dm; r; r &= r - 1) {
+                const uint8_t* src = scene_brick(sc, si, (uint32_t)std::countr_zero(r));
+                bytes.insert(bytes.end(), src, src + 512);
+            }
+        }
+        if (!idx.empty()) {
+            rc = vrt_world_sync_bricks(world, (uint32_t)idx.size(), idx.data(), alloc.data(), dirty.data(),
+                                       bytes.data(), stream);
+            if (rc != VRT_OK) return rc;  // the dirty set is kept for the next attempt
+            sc->dirty.clear();
+        }
+        // the palette goes up with every sync, as in SyncBuffers (GpuRenderer.cpp:100-102)
+        uint64_t pal[256];
+        if (sc->has_palette)
+            std::memcpy(pal, sc->palette, sizeof(pal));
+        else
+            for (uint32_t i = 0; i < 256; i++) pal[i] = (uint64_t)i * 0x01010101ull;
+        return vrt_world_set_palette(world, pal, stream);
+    });
 }
 
 void vrt_scene_destroy(vrt_scene* sc) { delete sc; }
 
 int vrt_world_from_scene(const vrt_scene* sc, void* stream, vrt_world** out) {
     if (!sc || !out) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_world_from_scene: null argument");
+    scene_pack_edits(const_cast<vrt_scene*>(sc));
     vrt_world_desc desc;
     std::memset(&desc, 0, sizeof(desc));
     desc.log2_size_xz = sc->log2_xz;
