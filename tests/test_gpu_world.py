@@ -372,9 +372,9 @@ def test_scene_edits_sync_through_the_dirty_set(vrt, oracle_mod, cuda):
     """The reference's editing loop: VoxelMap::Set marks DirtyLocs, SyncBuffers(VoxelMap&)
     consumes it (vrt_scene_set_voxel, vrt_world_sync_scene).  Each round of edits uploads
     exactly the dirty bricks still allocated, clears the dirty set, and leaves the world's casts
-    equal to the oracle's on the edited dense grid."""
+    equal to the oracle's over the scene's sector stream (allocated empty bricks included), every
+    hit on a solid voxel of the edited dense grid."""
     import voxelrt_amd as V
-    from test_oracle import dense_world
 
     rng = np.random.default_rng(23)
     vox = blob_world(128, seed=43, density=0.01)
@@ -412,9 +412,15 @@ def test_scene_edits_sync_through_the_dirty_set(vrt, oracle_mod, cuda):
         assert scene.dirty_sectors == 0
         i1 = world.info()
         assert i1.upload_brick_bytes - i0.upload_brick_bytes == 512 * n_up, f"round {rnd}"
-        ref = dense_world(oracle_mod, vox).cast(o, d, (0, 0, 0), 128)
+        # the oracle over the scene's own sector stream: Set allocates bricks also for 0, and an
+        # allocated empty brick changes the steps' LOD (not the hits), as in the reference
+        ref = oracle_mod.OracleWorld.from_stream(scene.stream()).cast(o, d, (0, 0, 0), 128)
         t, rec = cast_np(world, o, d)
         assert_hits_equal(t, rec, ref["t"], ref["rec"], f"round {rnd}")
+        # every hit is a solid voxel of the edited grid, with its material
+        hit = (rec[:, 3] & 0xFF) != 0
+        hx, hy, hz = rec[hit, 0], rec[hit, 1], rec[hit, 2]
+        assert hit.any() and np.array_equal(vox[hy, hz, hx], (rec[hit, 3] & 0xFF).astype(np.uint8))
         assert world.info().num_bricks == scene.stream()["nbricks"]
 
 
