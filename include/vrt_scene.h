@@ -70,6 +70,11 @@ int vrt_scene_spheres(uint32_t log2_size_xz, uint32_t log2_size_y, uint32_t coun
 /* A scene from a dense voxel grid [y][z][x] of size_y * size_xz * size_xz bytes. */
 int vrt_scene_from_dense(uint32_t log2_size_xz, uint32_t log2_size_y, const uint8_t* voxels, vrt_scene** out);
 
+/* The scene's sector stream.  Edits made with vrt_scene_set_voxel or
+ * vrt_scene_delete_empty_bricks are folded in first.  The pointers stay valid until the
+ * scene is destroyed or the next call that folds new edits in (this one, vrt_scene_save_cvox,
+ * vrt_world_from_scene after an edit): a caller that edits the scene takes the stream again.
+ * A scene is not safe to use from several threads at once. */
 int vrt_scene_get(const vrt_scene* scene, vrt_sector_stream* out);
 /* Voxel at absolute coordinates (0 outside the world). */
 uint8_t vrt_scene_voxel(const vrt_scene* scene, int32_t x, int32_t y, int32_t z);

@@ -46,6 +46,7 @@ class Scene:
 
     def __init__(self, handle: int):
         self._h = C.c_void_p(handle)
+        self._edited = False  # once edited, stream() copies (its arrays are re-packed on edits)
 
     @staticmethod
     def terrain(log2_xz: int, log2_y: int, seed: int, *, octaves=4, frequency=1 / 64, gain=0.5, base=32.0,
@@ -100,15 +101,19 @@ class Scene:
         return pal
 
     def stream(self) -> dict:
-        """Sector stream views into the scene's memory; the dict keeps the scene alive."""
+        """The sector stream: views into the scene's memory (the dict keeps the scene alive), or
+        copies once the scene has been edited, since the library re-packs the stream's arrays
+        when it folds edits in and earlier views would then point at freed memory."""
         s = L.SectorStream()
         L.check(L.lib().vrt_scene_get(self._h, C.byref(s)))
+        copy = self._edited
 
         def view(ptr, n, dt):
             if n == 0:
                 return np.zeros(0, dtype=dt)
             buf = (C.c_uint8 * (n * np.dtype(dt).itemsize)).from_address(ptr)
-            return np.frombuffer(buf, dtype=dt)
+            a = np.frombuffer(buf, dtype=dt)
+            return a.copy() if copy else a
 
         return {
             "log2_size_xz": s.log2_size_xz,
@@ -129,6 +134,7 @@ class Scene:
     def set_voxel(self, x: int, y: int, z: int, value: int) -> None:
         """VoxelMap::Set: writes one voxel, allocating its brick, and marks the brick dirty;
         positions outside the scene are ignored."""
+        self._edited = True
         L.check(L.lib().vrt_scene_set_voxel(self._h, x, y, z, value))
 
     def mark_all_dirty(self) -> None:
@@ -137,6 +143,7 @@ class Scene:
 
     def delete_empty_bricks(self) -> int:
         """Deletes bricks without a solid voxel (marking them dirty); returns how many."""
+        self._edited = True
         n = C.c_uint64()
         L.check(L.lib().vrt_scene_delete_empty_bricks(self._h, C.byref(n)))
         return int(n.value)
