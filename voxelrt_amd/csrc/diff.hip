@@ -154,8 +154,10 @@ __global__ __launch_bounds__(256) void render_bwd_kernel(uint32_t n, const float
 constexpr uint32_t kScanBlock = 1024;  // counts per block of the two-level scan
 
 // Pass 1: the rank of each contributing pixel among the pixels of its slot; counts[slot] ends
-// as the slot's pixel count.  Lanes of a wave with the same slot share one atomic (the first
-// pending lane's slot is matched by ballot; a lane's rank is its position among the matches).
+// as the slot's pixel count.  Lanes of a wave with the same slot share one atomic (each round
+// matches the first pending lane's slot by ballot; a lane's rank is its position among the
+// matches).  Eight rounds, unrolled with one result variable each, so a wave's atomics are all
+// in flight together and the wave waits for their returns once, after the last round.
 __global__ __launch_bounds__(256) void bwd_count_kernel(uint32_t n, const uint32_t* __restrict__ saved_voxel,
                                                         const float* __restrict__ saved_weight,
                                                         uint32_t* __restrict__ counts, uint32_t* __restrict__ rank) {
@@ -166,24 +168,44 @@ __global__ __launch_bounds__(256) void bwd_count_kernel(uint32_t n, const uint32
         if (gv != 0xFFFFFFFFu && saved_weight[i] != 0.0f) slot = gv >> 9;
     }
     bool pending = slot != 0xFFFFFFFFu;
-    uint32_t r = 0xFFFFFFFFu;
     const uint32_t lane = __lane_id();
-    for (int round = 0; round < 8; round++) {
+    uint32_t my_leader = 0;
+    uint64_t my_same = 0;
+    bool matched = false;
+    // one round: the leader's atomic result lands in its own variable b, read only after the
+    // last round
+    auto round = [&](uint32_t& b) {
         uint64_t todo = __ballot(pending);
-        if (todo == 0) break;
+        if (todo == 0) return;  // wave-uniform
         uint32_t leader = (uint32_t)__ffsll((unsigned long long)todo) - 1u;
         uint32_t key = __shfl(slot, (int)leader, 64);
         bool mine = pending && slot == key;
         uint64_t same = __ballot(mine);
-        uint32_t base = 0;
-        if (lane == leader) base = atomicAdd(&counts[key], (uint32_t)__popcll(same));
-        base = __shfl(base, (int)leader, 64);
+        if (lane == leader) b = atomicAdd(&counts[key], (uint32_t)__popcll(same));
         if (mine) {
-            r = base + __builtin_amdgcn_mbcnt_hi((uint32_t)(same >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)same, 0u));
+            my_leader = leader;
+            my_same = same;
+            matched = true;
             pending = false;
         }
-    }
-    if (pending) r = atomicAdd(&counts[slot], 1u);
+    };
+    uint32_t b0 = 0, b1 = 0, b2 = 0, b3 = 0, b4 = 0, b5 = 0, b6 = 0, b7 = 0;
+    round(b0);
+    round(b1);
+    round(b2);
+    round(b3);
+    round(b4);
+    round(b5);
+    round(b6);
+    round(b7);
+    // a lane leads at most one round and its other variables stay 0, so their OR is the base of
+    // the round it led
+    uint32_t led = b0 | b1 | b2 | b3 | b4 | b5 | b6 | b7;
+    uint32_t r = 0xFFFFFFFFu;
+    uint32_t b = __shfl(led, (int)my_leader, 64);
+    if (matched)
+        r = b + __builtin_amdgcn_mbcnt_hi((uint32_t)(my_same >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)my_same, 0u));
+    if (pending) r = atomicAdd(&counts[slot], 1u);  // slots left after the unrolled rounds
     if (i < n) rank[i] = r;
 }
 
@@ -262,6 +284,9 @@ __global__ __launch_bounds__(256) void bwd_place_kernel(uint32_t n, const uint32
 // Pass 4: one wave per slot.  The slot's pixels add grad_image * weight into 512 x 3 LDS
 // accumulators (LDS float atomics), then the wave writes the slot's rows [slot * 512,
 // min(nvox, slot * 512 + 512)) of the gradient: every entry is written, zeros where no pixel hit.
+// On the 512^3 terrain's 97 k slots this moves 598 MB of gradient and ~145 MB of gathered pixel
+// data in 0.133 ms (5.6 TB/s); four slots per workgroup (0.145 ms) and a grid stride over 8192
+// long-lived waves (0.156 ms) measured slower.
 __global__ __launch_bounds__(64) void bwd_reduce_kernel(uint64_t nvox, const float* __restrict__ grad_image,
                                                         const uint32_t* __restrict__ saved_voxel,
                                                         const float* __restrict__ saved_weight,
