@@ -424,6 +424,33 @@ def test_scene_edits_sync_through_the_dirty_set(vrt, oracle_mod, cuda):
         assert world.info().num_bricks == scene.stream()["nbricks"]
 
 
+def test_mark_all_dirty_fills_an_empty_world(vrt, oracle_mod, cuda):
+    """MarkAllDirty then SyncBuffers into an empty world (the reference's re-upload after a buffer
+    resize, GpuRenderer.cpp:85-97) gives the world vrt_world_from_scene builds; a world of other
+    extents is refused and the dirty set kept."""
+    import voxelrt_amd as V
+
+    vox = blob_world(128, seed=44, density=0.01)
+    scene = V.Scene.from_dense(vox)
+    scene.mark_all_dirty()
+    assert scene.dirty_sectors == len(scene.stream()["sector_idx"])
+    with pytest.raises(V.VrtError, match="extents"):
+        V.World(6, 6).sync_scene(scene)
+    assert scene.dirty_sectors > 0
+    world = V.World(7, 7)
+    world.sync_scene(scene)
+    assert scene.dirty_sectors == 0
+    st = scene.stream()
+    assert world.info().num_bricks == st["nbricks"]
+    assert world.info().upload_brick_bytes == 512 * st["nbricks"]
+    o, d = random_rays(20000, 128, seed=45)
+    t, rec = cast_np(world, o, d)
+    t2, rec2 = cast_np(V.World.from_scene(scene), o, d)
+    assert np.array_equal(rec, rec2) and np.array_equal(t.view(np.uint32), t2.view(np.uint32))
+    ref = oracle_mod.OracleWorld.from_stream(st).cast(o, d, (0, 0, 0), 128)
+    assert_hits_equal(t, rec, ref["t"], ref["rec"], "mark all dirty")
+
+
 def test_sync_staging_failure_leaves_world_unchanged(vrt, cuda):
     """The staging buffers are allocated by a world's first sync, before any device work: a failure
     there returns an error and the world stays as it was (empty), usable by the next sync."""
