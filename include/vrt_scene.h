@@ -136,8 +136,12 @@ int vrt_scene_load_cvox(const char* path, uint32_t log2_size_xz, uint32_t log2_s
                         const int32_t sector_offset[3], vrt_scene** out, vrt_cvox_info* info);
 int vrt_scene_save_cvox(const vrt_scene* scene, const int32_t sector_offset[3], const uint8_t* materials,
                         const char* path);
-/* The scene's encoded palette (a loaded file's); VRT_ERR_STATE for generated scenes. */
+/* The scene's encoded palette (a loaded file's, or one set below); VRT_ERR_STATE for a
+ * generated scene without one. */
 int vrt_scene_get_palette(const vrt_scene* scene, uint64_t palette[256]);
+/* Sets the scene's palette (VoxelMap::Palette, VoxelMap.h:187, encoded as
+ * Material::GetEncoded): vrt_world_sync_scene uploads it with every sync. */
+int vrt_scene_set_palette(vrt_scene* scene, const uint64_t palette[256]);
 void vrt_scene_destroy(vrt_scene* scene);
 
 /* Convenience: vrt_world_create + vrt_world_sync of the whole scene. */
@@ -148,8 +152,9 @@ int vrt_world_from_scene(const vrt_scene* scene, void* stream, vrt_world** out);
  * the scene's dirty set.  Every dirty sector is synced with the scene's current allocation
  * mask (0 for a sector the scene no longer holds) and its dirty bricks that are still
  * allocated (dirtyMask &= allocMask, GpuRenderer.cpp:62), through vrt_world_sync_bricks;
- * then the scene's palette is uploaded, on every call as SyncBuffers does
- * (GpuRenderer.cpp:100-102).  The world's extents must equal the scene's (the same sector
+ * then the scene's palette, if it has one (a loaded file's or vrt_scene_set_palette's), is
+ * uploaded, on every call as SyncBuffers does (GpuRenderer.cpp:100-102); without one the
+ * world's palette is left as it is.  The world's extents must equal the scene's (the same sector
  * indices).  On success the dirty set is cleared; on failure it is kept.
  */
 int vrt_world_sync_scene(vrt_world* world, vrt_scene* scene, void* stream);

@@ -427,7 +427,8 @@ def test_scene_edits_sync_through_the_dirty_set(vrt, oracle_mod, cuda):
 def test_mark_all_dirty_fills_an_empty_world(vrt, oracle_mod, cuda):
     """MarkAllDirty then SyncBuffers into an empty world (the reference's re-upload after a buffer
     resize, GpuRenderer.cpp:85-97) gives the world vrt_world_from_scene builds; a world of other
-    extents is refused and the dirty set kept."""
+    extents is refused and the dirty set kept; the scene's palette, once set, goes up with each
+    sync (GpuRenderer.cpp:100-102), and a scene without one leaves the world's."""
     import voxelrt_amd as V
 
     vox = blob_world(128, seed=44, density=0.01)
@@ -449,6 +450,23 @@ def test_mark_all_dirty_fills_an_empty_world(vrt, oracle_mod, cuda):
     assert np.array_equal(rec, rec2) and np.array_equal(t.view(np.uint32), t2.view(np.uint32))
     ref = oracle_mod.OracleWorld.from_stream(st).cast(o, d, (0, 0, 0), 128)
     assert_hits_equal(t, rec, ref["t"], ref["rec"], "mark all dirty")
+
+    # palette: a scene without one leaves the world's; a scene's palette goes up with each sync
+    def material_data():
+        h = world.cast(V.RayBatch.from_numpy(o, d), 128, with_material=True)
+        torch.cuda.synchronize()
+        return h.rec[:, 3].cpu().numpy() & 0xFF, h.material_data.cpu().numpy().astype(np.uint32)
+
+    mine = (np.arange(256, dtype=np.uint64) * np.uint64(0x9E3779B1)) & np.uint64(0xFFFFFFFF)
+    world.set_palette(mine)
+    world.sync_scene(scene)
+    mat, md = material_data()
+    assert np.array_equal(md[mat != 0], mine[mat[mat != 0]].astype(np.uint32))
+    theirs = (np.arange(256, dtype=np.uint64) * np.uint64(0x85EBCA6B) + np.uint64(7)) & np.uint64(0xFFFFFFFF)
+    scene.set_palette(theirs)
+    world.sync_scene(scene)
+    mat, md = material_data()
+    assert np.array_equal(md[mat != 0], theirs[mat[mat != 0]].astype(np.uint32))
 
 
 def test_sync_staging_failure_leaves_world_unchanged(vrt, cuda):

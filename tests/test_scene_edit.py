@@ -103,3 +103,12 @@ def test_edited_scene_round_trips_through_cvox(vrt, tmp_path):
     back, _ = vrt.Scene.load_cvox(str(path), 6, 6)
     for p in rng.integers(0, 64, size=(4000, 3)):
         assert back.voxel(*p) == vox[p[1], p[2], p[0]]
+
+
+def test_scene_palette_round_trip(vrt):
+    sc = vrt.Scene.from_dense(blob_world(64, seed=14, density=0.01))
+    with pytest.raises(vrt.VrtError):
+        sc.palette()  # a generated scene has none
+    pal = (np.arange(256, dtype=np.uint64) * np.uint64(0x0101010101)) ^ np.uint64(0xABCDEF)
+    sc.set_palette(pal)
+    assert np.array_equal(sc.palette(), pal)

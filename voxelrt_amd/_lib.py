@@ -204,6 +204,7 @@ _SIGNATURES = {
     "vrt_scene_load_cvox": (C.c_int, [C.c_char_p, u32, u32, C.POINTER(i32 * 3), C.POINTER(P), C.POINTER(CvoxInfo)]),
     "vrt_scene_save_cvox": (C.c_int, [P, C.POINTER(i32 * 3), P, C.c_char_p]),
     "vrt_scene_get_palette": (C.c_int, [P, P]),
+    "vrt_scene_set_palette": (C.c_int, [P, P]),
     "vrt_world_from_scene": (C.c_int, [P, P, C.POINTER(P)]),
     "vrt_world_sync_scene": (C.c_int, [P, P, P]),
     "vrt_camera_look": (C.c_int, [C.POINTER(C.c_double * 3), C.c_double, C.c_double, C.c_double, u32, u32, u32, u32,

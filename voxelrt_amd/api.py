@@ -95,10 +95,17 @@ class Scene:
                                             m.ctypes.data if m is not None else None, str(path).encode()))
 
     def palette(self) -> np.ndarray:
-        """Encoded palette of a loaded file (Material::GetEncoded per entry)."""
+        """The scene's encoded palette (Material::GetEncoded per entry): a loaded file's or one
+        set with set_palette."""
         pal = np.zeros(256, dtype=np.uint64)
         L.check(L.lib().vrt_scene_get_palette(self._h, pal.ctypes.data))
         return pal
+
+    def set_palette(self, palette: np.ndarray) -> None:
+        """VoxelMap::Palette: uploaded by every World.sync_scene."""
+        p = np.ascontiguousarray(palette, dtype=np.uint64)
+        assert p.shape == (256,)
+        L.check(L.lib().vrt_scene_set_palette(self._h, p.ctypes.data))
 
     def stream(self) -> dict:
         """The sector stream: views into the scene's memory (the dict keeps the scene alive), or

@@ -375,4 +375,13 @@ int vrt_scene_get_palette(const vrt_scene* sc, uint64_t palette[256]) {
     });
 }
 
+int vrt_scene_set_palette(vrt_scene* sc, const uint64_t palette[256]) {
+    return guarded([&]() -> int {
+        if (!sc || !palette) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_scene_set_palette: null argument");
+        std::memcpy(sc->palette, palette, sizeof(sc->palette));
+        sc->has_palette = true;
+        return VRT_OK;
+    });
+}
+
 }  // extern "C"

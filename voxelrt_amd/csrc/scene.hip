@@ -547,13 +547,10 @@ int vrt_world_sync_scene(vrt_world* world, vrt_scene* sc, void* stream) {
             if (rc != VRT_OK) return rc;  // the dirty set is kept for the next attempt
             sc->dirty.clear();
         }
-        // the palette goes up with every sync, as in SyncBuffers (GpuRenderer.cpp:100-102)
-        uint64_t pal[256];
-        if (sc->has_palette)
-            std::memcpy(pal, sc->palette, sizeof(pal));
-        else
-            for (uint32_t i = 0; i < 256; i++) pal[i] = (uint64_t)i * 0x01010101ull;
-        return vrt_world_set_palette(world, pal, stream);
+        // the scene's palette goes up with every sync, as VoxelMap::Palette does in SyncBuffers
+        // (GpuRenderer.cpp:100-102); a scene without one leaves the world's palette as it is
+        if (!sc->has_palette) return VRT_OK;
+        return vrt_world_set_palette(world, sc->palette, stream);
     });
 }
 
