@@ -469,6 +469,98 @@ def test_mark_all_dirty_fills_an_empty_world(vrt, oracle_mod, cuda):
     assert np.array_equal(md[mat != 0], theirs[mat[mat != 0]].astype(np.uint32))
 
 
+def _device_bytes(ptr: int, nbytes: int) -> np.ndarray:
+    torch.cuda.synchronize()
+    return torch.as_tensor(_DeviceWords(ptr, (nbytes // 4,)), device="cuda").cpu().numpy().view(np.uint8)
+
+
+def _cell_masks(brick: np.ndarray) -> list[int]:
+    """The 8 occupancy masks of a brick [y][z][x] (8^3 bytes), by brute force: cell cx + 2cz + 4cy,
+    bit vx + 4vz + 16vy (CpuRenderer.cpp:63-83)."""
+    out = []
+    for cy in range(2):
+        for cz in range(2):
+            for cx in range(2):
+                m = 0
+                for vy in range(4):
+                    for vz in range(4):
+                        for vx in range(4):
+                            if brick[cy * 4 + vy, cz * 4 + vz, cx * 4 + vx]:
+                                m |= 1 << (vx + 4 * vz + 16 * vy)
+                out.append(m)  # appended in cell order cx + 2cz + 4cy
+    return out
+
+
+def test_device_layout_matches_brute_force(vrt, cuda):
+    """The device world read back after incremental syncs that move bricks: every allocated brick
+    sits in the slot an explicit per-brick slot table gives (bricks of a sector in increasing
+    index from its base slot, BrickSlotAllocator.h:37-41), holds its voxel bytes, and its 8
+    occupancy masks equal a brute-force build from those bytes; masks split at bit 32 included."""
+    import voxelrt_amd as V
+
+    rng = np.random.default_rng(29)
+    vox = blob_world(128, seed=47, density=0.01)
+    vox[64:96, 0:32, 0:32] = 0
+    vox[80:96, 0:32, 0:32] = np.where(rng.random((16, 32, 32)) < 0.05, 9, 0)  # bricks 32..63 only
+    vox[0:32, 96:128, 96:128] = 0
+    vox[0:16, 96:128, 96:128] = np.where(rng.random((16, 32, 32)) < 0.05, 11, 0)  # bricks 0..31 only
+    world = V.World(7, 7)
+    world.sync(*sector_stream_of(vox, list(range(64))))
+    split_sectors = {(0, 2, 0), (3, 0, 3)}  # (x, y, z) of the two sectors filled above: left alone
+    for _ in range(3):  # edits that grow, shrink and move sectors' slot runs
+        done = 0
+        while done < 6:
+            x, y, z = (int(c) for c in rng.integers(0, 120, 3))
+            touched = {(a >> 5, b >> 5, c >> 5) for a in (x, x + 7) for b in (y, y + 7) for c in (z, z + 7)}
+            if touched & split_sectors:
+                continue
+            before = vox.copy()
+            vox[y:y + 8, z:z + 8, x:x + 8] = 0 if rng.random() < 0.5 else int(rng.integers(1, 256))
+            done += 1
+            # only the bricks that changed are dirty, so the bricks kept around them move when
+            # their sector's slot run changes
+            dirty = {}
+            for bz in range(z >> 3, ((z + 7) >> 3) + 1):
+                for by in range(y >> 3, ((y + 7) >> 3) + 1):
+                    for bx in range(x >> 3, ((x + 7) >> 3) + 1):
+                        sl = np.s_[by * 8:by * 8 + 8, bz * 8:bz * 8 + 8, bx * 8:bx * 8 + 8]
+                        if not np.array_equal(before[sl], vox[sl]):
+                            si, bi = brick_of(bx * 8, by * 8, bz * 8)
+                            dirty[si] = dirty.get(si, 0) | 1 << bi
+            if dirty:
+                world.sync_bricks(*dirty_stream(vox, dirty))
+    info = world.info()
+    buf = world.buffers()
+    rec = sector_records(world)
+    occ = _device_bytes(buf.occupancy, info.slot_capacity * 64).view(np.uint64)
+    dev_vox = _device_bytes(buf.voxels, info.slot_capacity * 512)
+    used = set()
+    split = {"lo": 0, "hi": 0}
+    for si in range(64):
+        mask = int(rec[si, 0]) | int(rec[si, 1]) << 32
+        base = int(rec[si, 2])
+        sx, sz, sy = si & 3, (si >> 2) & 3, si >> 4
+        sec = vox[sy * 32:(sy + 1) * 32, sz * 32:(sz + 1) * 32, sx * 32:(sx + 1) * 32]
+        slot = base
+        for b in range(64):
+            bx, bz, by = b & 3, (b >> 2) & 3, b >> 4
+            brick = sec[by * 8:(by + 1) * 8, bz * 8:(bz + 1) * 8, bx * 8:(bx + 1) * 8]
+            if not (mask >> b) & 1:
+                assert not brick.any(), (si, b)
+                continue
+            assert slot not in used, (si, b, slot)
+            used.add(slot)
+            assert np.array_equal(dev_vox[slot * 512:(slot + 1) * 512], brick.reshape(-1)), (si, b)
+            assert [int(m) for m in occ[slot * 8:(slot + 1) * 8]] == _cell_masks(brick), (si, b)
+            slot += 1
+        if mask and mask >> 32 == 0:
+            split["lo"] += 1
+        if mask and mask & 0xFFFFFFFF == 0:
+            split["hi"] += 1
+    assert split["lo"] > 0 and split["hi"] > 0 and world.info().moved_bricks > 0
+    assert len(used) == info.num_bricks
+
+
 def test_sync_staging_failure_leaves_world_unchanged(vrt, cuda):
     """The staging buffers are allocated by a world's first sync, before any device work: a failure
     there returns an error and the world stays as it was (empty), usable by the next sync."""
