@@ -112,3 +112,58 @@ def test_scene_palette_round_trip(vrt):
     pal = (np.arange(256, dtype=np.uint64) * np.uint64(0x0101010101)) ^ np.uint64(0xABCDEF)
     sc.set_palette(pal)
     assert np.array_equal(sc.palette(), pal)
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_random_edit_sequences_track_the_dense_grid(vrt, seed):
+    """Random interleavings of Set, empty-brick collection, MarkAllDirty and stream reads (each of
+    which folds the edits in): Get, the stream and the dirty set stay consistent with a dense
+    grid and a dirty set kept alongside."""
+    rng = np.random.default_rng(seed)
+    vox = blob_world(64, seed=20 + seed, density=0.01)
+    sc = vrt.Scene.from_dense(vox)
+    alloc = {}  # sector -> allocation mask, mirrored from the stream
+    dirty = {}
+
+    def mirror_alloc():
+        st = sc.stream()
+        alloc.clear()
+        alloc.update(zip(st["sector_idx"].tolist(), st["alloc_mask"].tolist()))
+        return st
+
+    mirror_alloc()
+    for _ in range(40):
+        op = rng.random()
+        if op < 0.7:
+            for _ in range(int(rng.integers(1, 60))):
+                x, y, z = (int(c) for c in rng.integers(-4, 68, 3))
+                v = int(rng.integers(0, 256)) if rng.random() < 0.7 else 0
+                sc.set_voxel(x, y, z, v)
+                if 0 <= x < 64 and 0 <= y < 64 and 0 <= z < 64:
+                    vox[y, z, x] = v
+                    si = sector_of(x, y, z, 64)
+                    bi = ((x >> 3) & 3) | ((z >> 3) & 3) << 2 | ((y >> 3) & 3) << 4
+                    dirty[si] = dirty.get(si, 0) | 1 << bi
+                    alloc[si] = alloc.get(si, 0) | 1 << bi
+        elif op < 0.8:
+            st = sc.stream()
+            before = dict(zip(st["sector_idx"].tolist(), st["alloc_mask"].tolist()))
+            sc.delete_empty_bricks()
+            st = mirror_alloc()
+            for si, m in before.items():
+                gone = m & ~alloc.get(si, 0)
+                if gone:
+                    dirty[si] = dirty.get(si, 0) | gone
+        elif op < 0.85:
+            sc.mark_all_dirty()
+            for si, m in alloc.items():
+                if m:
+                    dirty[si] = m
+        else:
+            st = mirror_alloc()
+            assert st["solid_voxels"] == int((vox != 0).sum())
+        assert sc.dirty_sectors == len(dirty)
+        for p in rng.integers(0, 64, size=(200, 3)):
+            assert sc.voxel(*p) == vox[p[1], p[2], p[0]]
+    sc.delete_empty_bricks()
+    assert streams_equal(sc.stream(), vrt.Scene.from_dense(vox).stream())
