@@ -88,3 +88,24 @@ def test_gbuffer_odd_size_and_errors(vrt, oracle_mod, cuda):
     o = L.RenderOpts(1, 0, 0, 2, (C.c_float * 3)(*SKY), 0.0, 0.0)  # unknown tile order
     assert vrt.lib().vrt_render_frame(world.handle, C.byref(cam.c), C.byref(o), C.byref(c), None) == \
         L.VRT_ERR_INVALID_ARGUMENT
+
+
+@pytest.mark.parametrize("bounces,lut", [(1, False), (2, True)])
+def test_c2_world_gbuffer(vrt, oracle_mod, cuda, bounces, lut):
+    """The G-buffer frame over the 1024^3 terrain + caves world (960x540, camera at floor(eye)):
+    the three images bit-exact with the oracle's."""
+    from voxelrt_amd import workloads as W
+    import oracle
+
+    sc = W.C2_WORLD.build()
+    world = vrt.World.from_scene(sc)
+    ow = oracle.OracleWorld.from_stream(sc.stream())
+    pal = random_palette(6)
+    world.set_palette(pal)
+    ow.set_palette(pal)
+    cam = W.c2_camera(tile=(1, 1), width=960, height=540, anchor="floor")
+    g, ref = render_both(vrt, world, ow, cam, bounces, 3, lut)
+    assert np.array_equal(g.albedo_normal.cpu().numpy().view(np.uint32), ref["albedo_normal"])
+    assert np.array_equal(g.irradiance.cpu().numpy().view(np.uint32), ref["irradiance"].view(np.uint32))
+    assert np.array_equal(g.depth.cpu().numpy().view(np.uint32), ref["depth"].view(np.uint32))
+    assert 0.3 < (ref["depth"] != -1.0).mean() < 1.0

@@ -103,3 +103,25 @@ def test_non_cubic_world_shader_parity(vrt, oracle_mod, cuda, size, ysize, seed)
             for method, lut in MODES:
                 rec = check_mode(world, ow, oo, d, W, method, lut, f"{size}x{ysize} inside={inside} W={W}")
                 assert ((rec[:, 3] & 0xFF) != 0).mean() > 0.05
+
+
+@pytest.fixture(scope="module")
+def c2_pair(vrt, oracle_mod):
+    from voxelrt_amd import workloads as W
+
+    sc = W.C2_WORLD.build()
+    return vrt.World.from_scene(sc), oracle_mod.OracleWorld.from_stream(sc.stream())
+
+
+@pytest.mark.parametrize("anchor", ["sector", "floor"])
+def test_c2_camera_full_size(vrt, cuda, c2_pair, anchor):
+    """The 1024^3 terrain + caves world and its 1920x1080 camera: every ray of all four modes
+    bit-exact with the oracle (records, t, hit positions), at both ray anchors."""
+    from voxelrt_amd import workloads as W
+
+    world, ow = c2_pair
+    cam = W.c2_camera(anchor=anchor)
+    o, d = cam.rays_host()
+    for method, lut in MODES:
+        rec = check_mode(world, ow, o, d, cam.world_origin, method, lut, f"C2 {anchor}")
+        assert ((rec[:, 3] & 0xFF) != 0).mean() > 0.3
