@@ -97,6 +97,11 @@ uint8_t vrt_scene_voxel(const vrt_scene* scene, int32_t x, int32_t y, int32_t z)
  */
 int vrt_scene_set_voxel(vrt_scene* scene, int32_t x, int32_t y, int32_t z, uint8_t value);
 
+/* n writes in order, each exactly as vrt_scene_set_voxel: xyz holds x, y, z of write k at
+ * 3k .. 3k + 2, values[k] its material (a region edit, as VoxelMap::RegionDispatchSIMD makes,
+ * VoxelMap.h:223-263, without a call per voxel). */
+int vrt_scene_set_voxels(vrt_scene* scene, uint64_t n, const int32_t* xyz, const uint8_t* values);
+
 /* VoxelMap::MarkAllDirty (VoxelMap.h:207-211): every sector of the scene becomes dirty
  * with its whole allocation mask. */
 int vrt_scene_mark_all_dirty(vrt_scene* scene);

@@ -167,3 +167,22 @@ def test_random_edit_sequences_track_the_dense_grid(vrt, seed):
             assert sc.voxel(*p) == vox[p[1], p[2], p[0]]
     sc.delete_empty_bricks()
     assert streams_equal(sc.stream(), vrt.Scene.from_dense(vox).stream())
+
+
+def test_batch_writes_equal_single_writes(vrt):
+    """set_voxels (one call, in order, repeated positions and outside positions included) leaves
+    the scene, its stream and its dirty set exactly as the same writes one by one."""
+    rng = np.random.default_rng(31)
+    vox = blob_world(64, seed=15, density=0.01)
+    a, b = vrt.Scene.from_dense(vox), vrt.Scene.from_dense(vox)
+    xyz = rng.integers(-3, 67, size=(5000, 3)).astype(np.int32)
+    xyz[2500:2600] = xyz[:100]  # rewritten positions: the later write wins
+    vals = rng.integers(0, 256, 5000).astype(np.uint8)
+    a.set_voxels(xyz, vals)
+    for (x, y, z), v in zip(xyz, vals):
+        b.set_voxel(int(x), int(y), int(z), int(v))
+    assert a.dirty_sectors == b.dirty_sectors
+    assert streams_equal(a.stream(), b.stream())
+    a.set_voxels(xyz[:10], 0)  # one value for all
+    for x, y, z in xyz[:10]:
+        assert a.voxel(int(x), int(y), int(z)) == 0

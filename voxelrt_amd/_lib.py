@@ -197,6 +197,7 @@ _SIGNATURES = {
     "vrt_scene_get": (C.c_int, [P, C.POINTER(SectorStream)]),
     "vrt_scene_voxel": (C.c_uint8, [P, i32, i32, i32]),
     "vrt_scene_set_voxel": (C.c_int, [P, i32, i32, i32, C.c_uint8]),
+    "vrt_scene_set_voxels": (C.c_int, [P, u64, P, P]),
     "vrt_scene_mark_all_dirty": (C.c_int, [P]),
     "vrt_scene_delete_empty_bricks": (C.c_int, [P, C.POINTER(C.c_uint64)]),
     "vrt_scene_dirty_sectors": (u32, [P]),

@@ -144,6 +144,14 @@ class Scene:
         self._edited = True
         L.check(L.lib().vrt_scene_set_voxel(self._h, x, y, z, value))
 
+    def set_voxels(self, xyz: np.ndarray, values) -> None:
+        """Many set_voxel writes in one call, in order: xyz [n, 3] (x, y, z), values [n] or one
+        value for all."""
+        p = np.ascontiguousarray(xyz, dtype=np.int32).reshape(-1, 3)
+        v = np.ascontiguousarray(np.broadcast_to(np.asarray(values, dtype=np.uint8), (p.shape[0],)))
+        self._edited = True
+        L.check(L.lib().vrt_scene_set_voxels(self._h, p.shape[0], p.ctypes.data, v.ctypes.data))
+
     def mark_all_dirty(self) -> None:
         """VoxelMap::MarkAllDirty: every sector dirty with its whole allocation mask."""
         L.check(L.lib().vrt_scene_mark_all_dirty(self._h))

@@ -467,6 +467,35 @@ int vrt_scene_set_voxel(vrt_scene* sc, int32_t x, int32_t y, int32_t z, uint8_t 
     });
 }
 
+int vrt_scene_set_voxels(vrt_scene* sc, uint64_t n, const int32_t* xyz, const uint8_t* values) {
+    return guarded([&]() -> int {
+        if (!sc) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_scene_set_voxels: null scene");
+        if (n && (!xyz || !values)) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_scene_set_voxels: null argument");
+        // the writes in order, each as vrt_scene_set_voxel; consecutive writes to one sector reuse
+        // its overlay entry
+        uint32_t last_si = UINT32_MAX;
+        vrt_edit_sector* e = nullptr;
+        uint64_t* dirty = nullptr;
+        for (uint64_t k = 0; k < n; k++) {
+            int32_t x = xyz[3 * k], y = xyz[3 * k + 1], z = xyz[3 * k + 2];
+            if (!((uint32_t)(x | z) < (1u << sc->log2_xz) && (uint32_t)y < (1u << sc->log2_y))) continue;
+            uint32_t ux = (uint32_t)x, uy = (uint32_t)y, uz = (uint32_t)z;
+            uint32_t si = sector_of(sc, ux, uy, uz), b = brick_of(ux, uy, uz);
+            if (si != last_si) {
+                e = &edit_sector(sc, si);
+                dirty = &sc->dirty[si];
+                last_si = si;
+            }
+            e->mask |= 1ull << b;
+            uint8_t& v = e->bricks[(size_t)b * 512 + voxel_of(ux, uy, uz)];
+            sc->solid += (values[k] != 0) - (v != 0);
+            v = values[k];
+            *dirty |= 1ull << b;
+        }
+        return VRT_OK;
+    });
+}
+
 int vrt_scene_mark_all_dirty(vrt_scene* sc) {
     return guarded([&]() -> int {
         if (!sc) return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_scene_mark_all_dirty: null scene");
