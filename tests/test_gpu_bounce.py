@@ -53,6 +53,14 @@ def test_primary_and_bounce_parity(vrt, oracle_mod, cuda, config, c2_scene):
     assert_hits_equal(g_t, g_rec, ref["t"], ref["rec"], f"{config} primary")
     # every primary's hit position (0 on a miss), bit for bit
     assert np.array_equal(g_pos.view(np.uint32), ref["pos"].view(np.uint32))
+    # the other schedules and block order on the full set: the persistent queue (two chunk
+    # sizes) and the grid dispatched last block first
+    for sched, chunk, order in ((2, 256, 0), (2, 1024, 0), (1, 0, 1)):
+        h2 = world.cast(prim, W.MAX_ITERS, with_pos=True, schedule=sched, chunk=chunk, order=order)
+        torch.cuda.synchronize()
+        what = f"{config} primary schedule={sched} chunk={chunk} order={order}"
+        assert_hits_equal(h2.t.cpu().numpy(), h2.rec.cpu().numpy(), ref["t"], ref["rec"], what)
+        assert np.array_equal(h2.pos.cpu().numpy().view(np.uint32), ref["pos"].view(np.uint32)), what
     # the SIMD twin's positions equal the scalar oracle's on a slice of the rays
     ref_s = ow.cast(po, pd, prim.world_origin, W.MAX_ITERS, with_pos=True, begin=0, end=min(po.shape[1], 50000))
     assert np.array_equal(ref_s["pos"].view(np.uint32), ref["pos"][:, :ref_s["pos"].shape[1]].view(np.uint32))
