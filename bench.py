@@ -72,21 +72,29 @@ def _time_ms(fn, reps: int) -> float:
     return s.elapsed_time(e) / reps
 
 
+_TIMING_STREAM = None
+
+
 def _graph_time_ms(launch, reps: int, replays: int = 5) -> float:
     """GPU time per call of launch(stream_handle): `reps` calls captured in one graph, the
     graph replayed `replays` times between HIP events, so host launch overhead does not enter
     small launches.  launch must issue no memset and no synchronisation."""
     import torch
 
-    side = torch.cuda.Stream()
+    global _TIMING_STREAM
+    if _TIMING_STREAM is None:
+        _TIMING_STREAM = torch.cuda.Stream()  # one stream: per-stream library state is allocated once
+    side = _TIMING_STREAM
     side.wait_stream(torch.cuda.current_stream())
     with torch.cuda.stream(side):
         for _ in range(2):
             launch(side.cuda_stream)
     torch.cuda.synchronize()
     g = torch.cuda.CUDAGraph()
-    with torch.cuda.graph(g):
-        h = torch.cuda.current_stream().cuda_stream
+    # captured on the warm-up stream: per-stream state sized by the warm-up (the compact
+    # schedule's survivor buffer) is the captured launches' state
+    with torch.cuda.graph(g, stream=side):
+        h = side.cuda_stream
         for _ in range(reps):
             launch(h)
     g.replay()
@@ -139,17 +147,30 @@ def cast_section(world, ow, rb, name: str, max_iters: int, cpu_threads: int | No
     world.cast(rb, max_iters, out=hits, counters=ctr)
     torch.cuda.synchronize()
     c = ctr.cpu().numpy()
-    opts = L.CastOpts(1, 0, 0, 0, 0, 0, L.VRT_ORDER_FORWARD, 0)
 
-    def launch(stream):
-        L.check(lib.vrt_cast_ex(world.handle, C.byref(r_c), max_iters, C.byref(h_c), C.byref(opts), stream))
+    def timed(sched, cap):
+        opts = L.CastOpts(sched, 0, 0, 0, 0, 0, L.VRT_ORDER_FORWARD, cap)
 
-    ms = _graph_time_ms(launch, reps)
+        def launch(stream):
+            L.check(lib.vrt_cast_ex(world.handle, C.byref(r_c), max_iters, C.byref(h_c), C.byref(opts), stream))
+
+        return _graph_time_ms(launch, reps)
+
+    # the grid, and the compact schedule (a capped pass, then the survivors repacked) at four
+    # first-pass caps: the fastest is the section's launch
+    trials = {(L.VRT_SCHED_GRID, 0): timed(L.VRT_SCHED_GRID, 0)}
+    for cap in (24, 32, 48, 64):
+        trials[(L.VRT_SCHED_COMPACT, cap)] = timed(L.VRT_SCHED_COMPACT, cap)
+    (sched, cap), ms = min(trials.items(), key=lambda kv: kv[1])
+    world.cast(rb, max_iters, out=hits, schedule=sched, pass_cap=cap)  # the measured launch's hits
+    torch.cuda.synchronize()
     res = {
         "rays": rb.n,
         "value": round(rb.n / ms / 1e3, 1),
         "unit": "Mrays/s",
         "ms_per_launch": round(ms, 5),
+        "schedule": "grid" if sched == L.VRT_SCHED_GRID else f"compact, first pass {cap} steps",
+        "grid_ms_per_launch": round(trials[(L.VRT_SCHED_GRID, 0)], 5),
         "iters_per_ray": round(float(c[1]) / rb.n, 3),
         "descents_per_ray": round(float(c[2]) / rb.n, 3),
         "hit_fraction": round(float(c[3]) / rb.n, 4),
@@ -168,7 +189,8 @@ def cast_section(world, ow, rb, name: str, max_iters: int, cpu_threads: int | No
                   np.array_equal(hits.t.cpu().numpy().view(np.uint32), ref["t"].view(np.uint32)))
         res["cpu_value"] = round(rb.n / med / 1e6, 2)
         res["parity_with_cpu"] = ok
-    log(f"  {name}: {rb.n} rays, {res['value']} Mrays/s ({ms:.4f} ms), {res['iters_per_ray']} lookups/ray, "
+    log(f"  {name}: {rb.n} rays, {res['value']} Mrays/s ({ms:.4f} ms, {res['schedule']}; grid "
+        f"{res['grid_ms_per_launch']:.4f} ms), {res['iters_per_ray']} lookups/ray, "
         f"roofline {res['roofline']['achieved']} GB/s" +
         (f", CPU {res['cpu_value']} Mrays/s, parity {res['parity_with_cpu']}" if cpu_threads is not None else ""))
     return res

@@ -214,9 +214,19 @@ int vrt_cast(const vrt_world* world, const vrt_rays* rays, uint32_t max_iters,
 typedef enum vrt_cast_schedule {
     VRT_SCHED_AUTO = 0,        /* the default (GRID) */
     VRT_SCHED_GRID = 1,        /* one ray per lane, one lane per ray, grid covers n */
-    VRT_SCHED_PERSISTENT = 2   /* persistent waves pull ray chunks from a queue and refill idle lanes;
+    VRT_SCHED_PERSISTENT = 2,  /* persistent waves pull ray chunks from a queue and refill idle lanes;
                                   the queue counters belong to the launch's (device, stream) pair, so
                                   persistent casts on different streams never share them */
+    VRT_SCHED_COMPACT = 3      /* cap and compact: a grid pass stops every ray after pass_cap steps;
+                                  the rays still running are packed 64 to a wave and resumed from
+                                  their saved state by a second pass.  For incoherent ray sets, whose
+                                  lanes finish at very different steps.  The survivor state (24 bytes
+                                  per ray of capacity and a 64-bit ballot per 64 rays) belongs to the
+                                  (device, stream) pair; the first call on a stream, and a call with
+                                  more rays than any before on it, allocates it (with a stream
+                                  synchronization), so make one such call before capturing a graph.
+                                  A captured compact cast keeps using its capture stream's state:
+                                  replays must not overlap compact casts on that stream. */
 } vrt_cast_schedule;
 
 /* Order in which the grid schedule dispatches its blocks of 64 rays (results are identical). */
@@ -234,8 +244,9 @@ typedef struct vrt_cast_opts {
     vrt_cast_counters* counters;/* optional device pointer, accumulated (not cleared) */
     const uint32_t* count;      /* optional device pointer: rays to trace, read when the kernel
                                    runs (rays->n is then the capacity and the upper bound) */
-    uint32_t order;             /* grid: vrt_cast_order (0 = forward) */
-    uint32_t reserved;          /* 0 */
+    uint32_t order;             /* grid and compact (first pass): vrt_cast_order (0 = forward) */
+    uint32_t pass_cap;          /* compact: steps of the first pass (0 = 32); a cap at or above
+                                   max_iters makes the cast one grid pass */
 } vrt_cast_opts;
 
 int vrt_cast_ex(const vrt_world* world, const vrt_rays* rays, uint32_t max_iters,

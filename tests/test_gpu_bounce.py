@@ -76,6 +76,14 @@ def test_primary_and_bounce_parity(vrt, oracle_mod, cuda, config, c2_scene):
     # bounce casts
     rb = ow.cast_simd(oo, od, prim.world_origin, W.MAX_ITERS)
     assert_hits_equal(hb.t[:nb].cpu().numpy(), hb.rec[:nb].cpu().numpy(), rb["t"], rb["rec"], f"{config} bounce")
+    # the compact schedule on the bounce set (device count, capacity-sized first pass)
+    count = torch.tensor([nb], dtype=torch.int32, device="cuda")
+    for cap in (0, 24):
+        hc = world.cast(br, W.MAX_ITERS, schedule=3, pass_cap=cap, count=count)
+        torch.cuda.synchronize()
+        assert_hits_equal(hc.t[:nb].cpu().numpy(), hc.rec[:nb].cpu().numpy(), rb["t"], rb["rec"],
+                          f"{config} bounce compact cap={cap}")
+        assert np.array_equal(hc.t[:nb].cpu().numpy().view(np.uint32), rb["t"].view(np.uint32))
     # bounce rays start next to a solid face and go away from it: cosine-weighted around N
     n_code = g_rec[osrc, 3]
     N = np.stack([(((n_code >> (8 + 2 * a)) & 3) ^ 2) - 2 for a in range(3)]).astype(np.float64)
@@ -116,7 +124,7 @@ def test_device_count_bounds_the_cast(vrt, oracle_mod, cuda):
     scene, cam = W.C1_WORLD.build(), W.c1_camera()
     world = V.World.from_scene(scene)
     prim = cam.rays_device()
-    for sched in (1, 2):
+    for sched in (1, 2, 3):
         for k in (0, 1, 777, prim.n):
             cnt = torch.tensor([k], dtype=torch.int32, device="cuda")
             h = V.HitBatch.empty(prim.n)

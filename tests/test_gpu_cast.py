@@ -13,17 +13,20 @@ from test_oracle import dense_world
 pytestmark = pytest.mark.gpu
 
 GOLDEN = Path(__file__).resolve().parent / "golden"
-# (schedule, chunk): grid in forward and reverse block order; persistent with 64/256/1024-ray chunks
-SCHEDULES = [(1, 0), (-1, 0), (2, 64), (2, 256), (2, 1024)]
+# (schedule, parameter): grid in forward and reverse block order; persistent with 64/256/1024-ray
+# chunks; compact with first passes of 1, 7 and 32 steps (1: every survivor resumes after its
+# first step)
+SCHEDULES = [(1, 0), (-1, 0), (2, 64), (2, 256), (2, 1024), (3, 1), (3, 7), (3, 32)]
 
 
 def gpu_cast(world, o, d, W, max_iters=128, schedule=1, chunk=0, with_pos=False, counters=None):
     from voxelrt_amd import RayBatch
 
     rb = RayBatch.from_numpy(o, d, W)
-    # schedule -1: the grid with its blocks dispatched last first
-    h = world.cast(rb, max_iters, with_pos=with_pos, schedule=abs(schedule), chunk=chunk, counters=counters,
-                   order=1 if schedule < 0 else 0)
+    # schedule -1: the grid with its blocks dispatched last first; chunk: the persistent
+    # schedule's chunk or the compact schedule's first-pass cap
+    h = world.cast(rb, max_iters, with_pos=with_pos, schedule=abs(schedule), chunk=chunk if schedule == 2 else 0,
+                   counters=counters, order=1 if schedule < 0 else 0, pass_cap=chunk if schedule == 3 else 0)
     torch.cuda.synchronize()
     return h.t.cpu().numpy(), h.rec.cpu().numpy(), (h.pos.cpu().numpy() if h.pos is not None else None)
 

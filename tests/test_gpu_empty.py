@@ -25,12 +25,12 @@ def test_zero_rays_every_schedule(vrt, cuda, small_world):
 
     world, _ = small_world
     empty = V.RayBatch.empty(0)
-    for sched, chunk in [(0, 0), (1, 0), (2, 64)]:
+    for sched, chunk in [(0, 0), (1, 0), (2, 64), (3, 0)]:
         ctr = V.new_counters()
         h = world.cast(empty, 128, with_pos=True, schedule=sched, chunk=chunk, counters=ctr)
         torch.cuda.synchronize()
         assert h.t.numel() == 0 and int(ctr.sum().item()) == 0
-    for bad in (dict(order=2), dict(schedule=9), dict(schedule=3), dict(schedule=5), dict(schedule=2, chunk=100)):
+    for bad in (dict(order=2), dict(schedule=9), dict(schedule=4), dict(schedule=5), dict(schedule=2, chunk=100)):
         with pytest.raises(V.VrtError):
             world.cast(empty, 128, **bad)
     with pytest.raises(V.VrtError):

@@ -201,6 +201,17 @@ def test_c3_spheres_parity(vrt, oracle_mod, cuda, sparse):
         rate = rb.n * 5 / (time.perf_counter() - t0) / 1e6
         ref = ow.cast_simd(rb.o.cpu().numpy(), rb.d.cpu().numpy(), rb.world_origin, W.MAX_ITERS)
         assert_hits_equal(h.t.cpu().numpy(), h.rec.cpu().numpy(), ref["t"], ref["rec"], f"C3 {name}")
+        # the compact schedule (first passes of 24, 32 and 48 steps) on the full sets, with the
+        # grid's lookup, descent and hit totals
+        ctr_g = V.new_counters()
+        world.cast(rb, W.MAX_ITERS, counters=ctr_g)
+        for cap in (24, 0, 48):
+            ctr = V.new_counters()
+            hc = world.cast(rb, W.MAX_ITERS, schedule=3, pass_cap=cap, counters=ctr)
+            torch.cuda.synchronize()
+            assert_hits_equal(hc.t.cpu().numpy(), hc.rec.cpu().numpy(), ref["t"], ref["rec"], f"C3 {name} cap={cap}")
+            assert np.array_equal(hc.t.cpu().numpy().view(np.uint32), ref["t"].view(np.uint32))
+            assert torch.equal(ctr, ctr_g), (cap, ctr.tolist(), ctr_g.tolist())
         hit = float(((ref["rec"][:, 3] & 0xFF) != 0).mean())
         print(f"C3 sparse={sparse} {name}: {rb.n} rays, {rate:.0f} Mrays/s, hit fraction {hit:.3f}", flush=True)
 

@@ -20,6 +20,7 @@ VRT_ERR_STATE = 5
 VRT_SCHED_AUTO = 0
 VRT_SCHED_GRID = 1
 VRT_SCHED_PERSISTENT = 2
+VRT_SCHED_COMPACT = 3
 
 VRT_ORDER_FORWARD = 0
 VRT_ORDER_REVERSE = 1
@@ -87,7 +88,7 @@ class CastCounters(C.Structure):
 
 class CastOpts(C.Structure):
     _fields_ = [("schedule", u32), ("refill", u32), ("chunk", u32), ("waves_per_cu", u32),
-                ("counters", P), ("count", P), ("order", u32), ("reserved", u32)]
+                ("counters", P), ("count", P), ("order", u32), ("pass_cap", u32)]
 
 
 class HitF64(C.Structure):

@@ -383,15 +383,17 @@ class World:
     def cast(self, rays: RayBatch, max_iters: int = 128, with_pos: bool = False, schedule: int = 0,
              chunk: int = 0, waves_per_cu: int = 0, counters: torch.Tensor | None = None,
              out: HitBatch | None = None, count: torch.Tensor | None = None, refill: int = 0,
-             order: int = 0, with_uv: bool = False, with_material: bool = False, stream=None) -> HitBatch:
+             order: int = 0, with_uv: bool = False, with_material: bool = False, stream=None,
+             pass_cap: int = 0) -> HitBatch:
         """Trace rays (RayCast, CpuRenderer.cpp:172-224).  count: optional device int32 holding the
         number of rays to trace (rays.n is then the capacity); order: the grid's block order
-        (VRT_ORDER_FORWARD / VRT_ORDER_REVERSE, same results)."""
+        (VRT_ORDER_FORWARD / VRT_ORDER_REVERSE, same results); pass_cap: the compact schedule's
+        first-pass steps (schedule 3, 0 = 32)."""
         hits = out if out is not None else HitBatch.empty(rays.n, with_pos, rays.o.device, with_uv, with_material)
         r = rays.to_c()
         h = hits.to_c()
         opts = L.CastOpts(schedule, refill, chunk, waves_per_cu, counters.data_ptr() if counters is not None else 0,
-                          count.data_ptr() if count is not None else 0, order, 0)
+                          count.data_ptr() if count is not None else 0, order, pass_cap)
         L.check(L.lib().vrt_cast_ex(self._h, C.byref(r), max_iters, C.byref(h), C.byref(opts),
                                     _stream_handle(stream)))
         return hits

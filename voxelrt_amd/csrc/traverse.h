@@ -314,12 +314,14 @@ __device__ __forceinline__ bool solid_voxel_at(const WorldView& w, const int32_t
 
 // RayCast's loop (CpuRenderer.cpp:185-202): steps until a hit, a miss or
 // max_iters steps; true on a hit, with v the hit voxel and slot its brick slot.
+// first: the steps the ray has already made (r.iters), for a ray resumed from a
+// saved position and side distances; the loop then makes at most max_iters - first.
 template <bool EXACT, bool CUBIC = false, int ORIGIN = ORIGIN_ANY>
 __device__ __forceinline__ bool ray_trace(const WorldView& w, RayState& r, uint32_t max_iters, int32_t v[3],
-                                          uint32_t& slot) {
+                                          uint32_t& slot, uint32_t first = 0) {
     bool hit = false;
     if (EXACT) {
-        for (uint32_t it = 0; it < max_iters; it++)
+        for (uint32_t it = first; it < max_iters; it++)
             if (!ray_advance<EXACT, CUBIC, ORIGIN>(w, r, v, hit, &slot)) break;
         return hit;
     }
@@ -328,7 +330,7 @@ __device__ __forceinline__ bool ray_trace(const WorldView& w, RayState& r, uint3
     // it was stepping over (every entry point rejects a cap of 0, so v is set by at least
     // one step); r.iters counts the steps that went on, max_iters only after the cap.
     uint32_t unused;
-    for (uint32_t it = 0; it < max_iters; it++)
+    for (uint32_t it = first; it < max_iters; it++)
         if (!ray_advance<EXACT, CUBIC, ORIGIN, true>(w, r, v, hit, &unused)) break;
     if (ORIGIN == ORIGIN_ANY) {
         // here the step overwrites v with the cell corner, and a test of the step count
@@ -348,10 +350,10 @@ __device__ __forceinline__ bool ray_trace(const WorldView& w, RayState& r, uint3
 
 template <bool CUBIC>
 __device__ __forceinline__ bool ray_trace_tame(const WorldView& w, RayState& r, uint32_t max_iters, int32_t v[3],
-                                               uint32_t& slot, int origin) {
-    if (origin == ORIGIN_ZERO) return ray_trace<false, CUBIC, ORIGIN_ZERO>(w, r, max_iters, v, slot);
-    if (origin == ORIGIN_ALIGNED) return ray_trace<false, CUBIC, ORIGIN_ALIGNED>(w, r, max_iters, v, slot);
-    return ray_trace<false, CUBIC>(w, r, max_iters, v, slot);
+                                               uint32_t& slot, int origin, uint32_t first) {
+    if (origin == ORIGIN_ZERO) return ray_trace<false, CUBIC, ORIGIN_ZERO>(w, r, max_iters, v, slot, first);
+    if (origin == ORIGIN_ALIGNED) return ray_trace<false, CUBIC, ORIGIN_ALIGNED>(w, r, max_iters, v, slot, first);
+    return ray_trace<false, CUBIC>(w, r, max_iters, v, slot, first);
 }
 
 // ray_trace with the step picked per wave: the fast step when every ray of the
@@ -359,10 +361,11 @@ __device__ __forceinline__ bool ray_trace_tame(const WorldView& w, RayState& r, 
 // and specialized for the rays' world origin (origin: an OriginKind, uniform over
 // the wave).
 __device__ __forceinline__ bool ray_trace_wave(const WorldView& w, RayState& r, uint32_t max_iters, int32_t v[3],
-                                               uint32_t& slot, bool wave_tame, int origin = ORIGIN_ANY) {
-    if (!wave_tame) return ray_trace<true>(w, r, max_iters, v, slot);
-    if (w.size_y == w.size_xz) return ray_trace_tame<true>(w, r, max_iters, v, slot, origin);
-    return ray_trace_tame<false>(w, r, max_iters, v, slot, origin);
+                                               uint32_t& slot, bool wave_tame, int origin = ORIGIN_ANY,
+                                               uint32_t first = 0) {
+    if (!wave_tame) return ray_trace<true>(w, r, max_iters, v, slot, first);
+    if (w.size_y == w.size_xz) return ray_trace_tame<true>(w, r, max_iters, v, slot, origin, first);
+    return ray_trace_tame<false>(w, r, max_iters, v, slot, origin, first);
 }
 
 // Hit record fields (vrt_hit in vrt.h) of a finished ray.
