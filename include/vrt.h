@@ -253,6 +253,15 @@ int vrt_cast_ex(const vrt_world* world, const vrt_rays* rays, uint32_t max_iters
                 vrt_hits* hits, const vrt_cast_opts* opts, void* stream);
 
 /*
+ * Frees the state the library keeps per (device, stream) for `stream` on every device: the
+ * persistent schedule's queue counters and the compact schedule's survivor state.  It
+ * synchronizes the stream first.  Call it before destroying a stream that ran such casts;
+ * a later cast on the stream allocates the state again.  A stream without state is a
+ * no-op.  No counterpart in the reference (its casts keep no per-stream state).
+ */
+int vrt_stream_release(void* stream);
+
+/*
  * Scalar FP64 cast of one ray on the device, the semantics of VoxelMap::RayCast
  * (VoxelMap.cpp:125-170): bias +0.0001, fixed 32/8/1 levels, normal from
  * tmin >= sideDist, distance = biased tmin.  Synchronous; host in/out.

@@ -134,3 +134,23 @@ def test_device_count(vrt, cuda, blob_pair, k):
     assert int(ctr[0].item()) == k
     assert np.array_equal(rec[:k], ref["rec"][:k])
     assert np.all(rec[k:] == -7)
+
+
+def test_release_stream_frees_and_later_casts_reallocate(vrt, cuda, blob_pair):
+    """vrt_stream_release frees a stream's compact and persistent state; casts on the stream
+    afterwards allocate it again and write the oracle's hits; releasing a stream without
+    state, or twice, is a no-op."""
+    import voxelrt_amd as V
+
+    world, ow = blob_pair
+    rb, ref = rays_and_ref(ow, 20000, seed=97)
+    s = torch.cuda.Stream()
+    for _ in range(2):
+        with torch.cuda.stream(s):
+            h1 = world.cast(rb, 128, with_pos=True, schedule=COMPACT, pass_cap=12, stream=s)
+            h2 = world.cast(rb, 128, with_pos=True, schedule=2, chunk=256, stream=s)
+        V.release_stream(s)
+        check(h1, ref, "compact before release")
+        check(h2, ref, "persistent before release")
+    V.release_stream(s)
+    V.release_stream(torch.cuda.Stream())

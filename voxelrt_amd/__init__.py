@@ -14,6 +14,7 @@ from .api import (  # noqa: F401
     CameraSpec,
     incoherent_rays_host,
     new_counters,
+    release_stream,
     bounce,
     cast_frame,
     render_frame,
@@ -29,7 +30,7 @@ from .api import (  # noqa: F401
 
 __all__ = [
     "VrtError", "CapacityError", "lib", "LIB_PATH", "Scene", "World", "RayBatch", "HitBatch", "HitResult",
-    "CameraSpec", "incoherent_rays_host", "new_counters", "bounce", "cast_frame", "render_frame",
+    "CameraSpec", "incoherent_rays_host", "new_counters", "release_stream", "bounce", "cast_frame", "render_frame",
     "GBufferImages", "Batch", "BatchPlan", "autotune", "render",
     "render_saved", "render_backward", "sun_direction",
 ]

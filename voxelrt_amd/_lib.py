@@ -171,6 +171,7 @@ _SIGNATURES = {
     "vrt_world_get_buffers": (C.c_int, [P, C.POINTER(WorldBuffers)]),
     "vrt_cast": (C.c_int, [P, C.POINTER(Rays), u32, C.POINTER(Hits), P]),
     "vrt_cast_ex": (C.c_int, [P, C.POINTER(Rays), u32, C.POINTER(Hits), C.POINTER(CastOpts), P]),
+    "vrt_stream_release": (C.c_int, [P]),
     "vrt_cast_one_f64": (C.c_int, [P, C.POINTER(C.c_double * 3), C.POINTER(C.c_double * 3), u32, C.POINTER(HitF64)]),
     "vrt_bounce_scratch_bytes": (C.c_size_t, [u32]),
     "vrt_bounce_generate": (C.c_int, [C.POINTER(Rays), C.POINTER(Hits), u32, u32, C.POINTER(RayBuffers), P, P, P, P]),

@@ -473,6 +473,12 @@ def render_frame(world: "World", cam: "CameraSpec", max_bounces: int = 1, frame:
     return g
 
 
+def release_stream(stream=None) -> None:
+    """vrt_stream_release: frees the library's per-stream cast state (persistent queue counters,
+    compact survivor state) of `stream` (torch's current stream when None), after synchronizing it."""
+    L.check(L.lib().vrt_stream_release(_stream_handle(stream)))
+
+
 def new_counters(device="cuda") -> torch.Tensor:
     """Zeroed vrt_cast_counters on the device (rays, iters, descents, hits)."""
     return torch.zeros(4, dtype=torch.int64, device=device)

@@ -662,6 +662,38 @@ int vrt_cast(const vrt_world* world, const vrt_rays* rays, uint32_t max_iters, v
     return vrt_cast_ex(world, rays, max_iters, hits, &o, stream);
 }
 
+int vrt_stream_release(void* stream) {
+    return guarded([&]() -> int {
+        const hipStream_t s = (hipStream_t)stream;
+        std::lock_guard<std::mutex> lk(g_dev_mu);
+        bool synced = false;
+        auto sync_once = [&]() -> int {
+            if (!synced) VRT_HIP(hipStreamSynchronize(s));
+            synced = true;
+            return VRT_OK;
+        };
+        for (auto it = g_queues.begin(); it != g_queues.end();) {
+            if (it->first.second != s) {
+                ++it;
+                continue;
+            }
+            if (int rc = sync_once(); rc != VRT_OK) return rc;
+            VRT_HIP(hipFree(it->second));
+            it = g_queues.erase(it);
+        }
+        for (auto it = g_survivors.begin(); it != g_survivors.end();) {
+            if (it->first.second != s) {
+                ++it;
+                continue;
+            }
+            if (int rc = sync_once(); rc != VRT_OK) return rc;
+            VRT_HIP(hipFree(it->second.base));
+            it = g_survivors.erase(it);
+        }
+        return VRT_OK;
+    });
+}
+
 int vrt_cast_ex(const vrt_world* world, const vrt_rays* rays, uint32_t max_iters, vrt_hits* hits,
                 const vrt_cast_opts* opts, void* stream) {
     return guarded([&]() -> int {
