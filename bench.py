@@ -16,12 +16,13 @@ on the launch stream), the same step with the camera rays anchored at
 floor(eye) (the reference's choice, CpuRenderer.cpp:447), the 256^3 terrain
 (640x360 primary rays) and the 2048^3 sphere worlds (sparse and literal:
 1920x1080 primary rays and 2M incoherent rays), the batched and
-differentiable forms, and the reference CPU path (the oracle's SIMD port of
-CpuRenderer.cpp:172-224) timed on this host's cores on the same ray buffers.
+differentiable forms, the GLSL-path cast and the renderer frame, and the
+reference CPU path (the oracle's SIMD port of CpuRenderer.cpp:172-224) timed
+on this host's cores on the same ray buffers.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--tile 8x8] [--schedule grid|persistent]
                     [--primary-order|--bounce-order|--fused-order auto|forward|reverse]
-                    [--anchor sector|floor] [--only all|batched|diff|c1|c3]
+                    [--anchor sector|floor] [--only all|batched|diff|c1|c3|extras]
 """
 from __future__ import annotations
 
@@ -339,6 +340,52 @@ def bench_batched(world, max_iters: int, dev) -> dict:
     return res
 
 
+# ---------------------------------------------------------------- shader method and renderer
+
+
+def bench_extras(world, max_iters: int) -> dict:
+    """The GLSL-path cast (vrt_cast_shader, all four modes) on the C2 camera rays and the GPU
+    renderer's frame (vrt_render_frame, 0/1/2 bounces) at 1920x1080 on the same world: GPU time
+    per launch from captured graphs (no host transfers, no allocation in the launches)."""
+    import voxelrt_amd as V
+    from voxelrt_amd import _lib as L
+    from voxelrt_amd import workloads as W
+
+    log("shader method and renderer frame on the 1024^3 terrain + caves world")
+    lib = V.lib()
+    cam = W.c2_camera()
+    rb = cam.rays_device()
+    hits = V.HitBatch.empty(rb.n)
+    r_c, h_c = rb.to_c(), hits.to_c()
+    shader = {}
+    for method in ("full", "coarse"):
+        for lut in (False, True):
+            o = L.ShaderOpts({"full": L.VRT_SHADER_FULL, "coarse": L.VRT_SHADER_COARSE}[method], int(lut), 0, 0)
+
+            def launch(stream, o=o):
+                L.check(lib.vrt_cast_shader(world.handle, C.byref(r_c), C.byref(o), C.byref(h_c), stream))
+
+            ms = _graph_time_ms(launch, 20)
+            key = f"{'rayCast' if method == 'full' else 'rayCastCoarse'}{', ray-cell LUT' if lut else ''}"
+            shader[key] = {"ms_per_launch": round(ms, 5), "value": round(rb.n / ms / 1e3, 1), "unit": "Mrays/s"}
+            log(f"  {key}: {shader[key]['value']} Mrays/s ({ms:.4f} ms, {rb.n} rays)")
+    frames = {}
+    g = V.GBufferImages(cam.c.width, cam.c.height)
+    g_c = g.to_c()
+    for bounces in (0, 1, 2):
+        o = L.RenderOpts(bounces, 0, 0, 0, (C.c_float * 3)(0.6, 0.75, 1.0), 0.0, 0.0)
+
+        def launch(stream, o=o):
+            L.check(lib.vrt_render_frame(world.handle, C.byref(cam.c), C.byref(o), C.byref(g_c), stream))
+
+        ms = _graph_time_ms(launch, 10)
+        frames[f"{bounces} bounces"] = {"ms_per_frame": round(ms, 4),
+                                        "pixels_per_s": round(cam.c.width * cam.c.height / ms * 1e3)}
+        log(f"  renderer frame, {bounces} bounces: {ms:.4f} ms")
+    return {"camera": f"{cam.c.width}x{cam.c.height}, world origin {list(rb.world_origin)}",
+            "shader_method": shader, "render_frame": frames}
+
+
 # ---------------------------------------------------------------- differentiable
 
 
@@ -552,11 +599,12 @@ def main() -> int:
                          "aligned-origin step) or floor(eye)")
     ap.add_argument("--event-every", type=int, default=8,
                     help="HIP events around the launches of every n-th timed step (the roofline's launch times)")
-    ap.add_argument("--only", default="all", choices=["all", "batched", "diff", "c1", "c3"],
+    ap.add_argument("--only", default="all", choices=["all", "batched", "diff", "c1", "c3", "extras"],
                     help="run one section alone and print its result (profiling runs); the default runs the "
                          "metric with every section")
     ap.add_argument("--cpu-runs", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip the shader-method and renderer section")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-diff", action="store_true")
     ap.add_argument("--no-configs", action="store_true", help="skip the 256^3 and 2048^3 sections")
@@ -598,7 +646,10 @@ def main() -> int:
         else:
             log("building the 1024^3 terrain + caves world")
             world = V.World.from_scene(W.C2_WORLD.build())
-            res = bench_batched(world, W.MAX_ITERS, dev)
+            if args.only == "batched":
+                res = bench_batched(world, W.MAX_ITERS, dev)
+            else:
+                res = bench_extras(world, W.MAX_ITERS)
         print(json.dumps({"section": args.only, **res}), flush=True)
         return 0
 
@@ -755,6 +806,7 @@ def main() -> int:
 
     del st
     batched = None if args.no_batched else bench_batched(world, W.MAX_ITERS, dev)
+    extras = None if args.no_extras else bench_extras(world, W.MAX_ITERS)
     del world
     configs = None
     if not args.no_configs:
@@ -793,6 +845,7 @@ def main() -> int:
         "fused_frame": fused,
         "cpu_baseline": cpu,
         "batched": batched,
+        "shader_and_renderer": extras,
         "configs": configs,
         "differentiable": differentiable,
     }
