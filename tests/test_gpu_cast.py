@@ -263,7 +263,8 @@ def test_counting_variant_non_cubic_world(vrt, oracle_mod, cuda, W):
 def test_uv_and_material_data_match_oracle(vrt, oracle_mod, cuda, W):
     """The rest of VHitResult (CpuRenderer.cpp:210-221): UV bit-exact with the oracle's, and the
     palette entry's low 32 bits of the hit material, for every world-origin specialization and
-    both schedules; other entry points reject the fields."""
+    every schedule (the compact one writes its survivors' fields in its second pass); other
+    entry points reject the fields."""
     import voxelrt_amd as V
     from voxelrt_amd import _lib as L
 
@@ -276,8 +277,9 @@ def test_uv_and_material_data_match_oracle(vrt, oracle_mod, cuda, W):
     o = (o - np.array(W, np.float32)[:, None]).astype(np.float32)
     ref = ow.cast(o, d, W, 128, with_pos=True, with_uv=True)
     rb = V.RayBatch.from_numpy(o, d, W)
-    for sched, chunk in [(1, 0), (2, 64)]:
-        h = world.cast(rb, 128, with_pos=True, schedule=sched, chunk=chunk, with_uv=True, with_material=True)
+    for sched, chunk in [(1, 0), (2, 64), (3, 0), (3, 5)]:
+        h = world.cast(rb, 128, with_pos=True, schedule=sched, chunk=chunk if sched == 2 else 0,
+                       pass_cap=chunk if sched == 3 else 0, with_uv=True, with_material=True)
         torch.cuda.synchronize()
         assert_hits_equal(h.t.cpu().numpy(), h.rec.cpu().numpy(), ref["t"], ref["rec"], f"W={W} sched={sched}")
         assert np.array_equal(h.uv.cpu().numpy().view(np.uint32), ref["uv"].view(np.uint32))
