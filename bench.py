@@ -493,9 +493,14 @@ def c2_step(world, cam, args, K: int, order_arg: dict, event_every: int) -> dict
         o = cur["primary"] if o is None else o
         L.check(lib.vrt_cast_ex(world.handle, C.byref(r_prim), W.MAX_ITERS, C.byref(h_prim), C.byref(o), sh))
 
+    # bounce rays in primary order, or grouped by direction octant (vrt_bounce_generate_ex): the
+    # same rays, the same hit per ray; the order changes the bounce launch's wave coherence
+    gen_orders = {"primary": L.VRT_BOUNCE_PRIMARY_ORDER, "direction": L.VRT_BOUNCE_DIRECTION_ORDER}
+    cur["gen"] = gen_orders.get(args.bounce_generation, L.VRT_BOUNCE_DIRECTION_ORDER)
+
     def gen_bounce():
-        L.check(lib.vrt_bounce_generate(C.byref(r_prim), C.byref(h_prim), cam.c.width, W.BOUNCE_SEED,
-                                        C.byref(bufs_b), src.data_ptr(), count.data_ptr(), scratch.data_ptr(), sh))
+        L.check(lib.vrt_bounce_generate_ex(C.byref(r_prim), C.byref(h_prim), cam.c.width, W.BOUNCE_SEED, cur["gen"],
+                                           C.byref(bufs_b), src.data_ptr(), count.data_ptr(), scratch.data_ptr(), sh))
 
     def cast_bounce(o=None):
         o = cur["bounce"] if o is None else o
@@ -508,18 +513,23 @@ def c2_step(world, cam, args, K: int, order_arg: dict, event_every: int) -> dict
     torch.cuda.synchronize()
     nb = int(count.item())
     trial = {}
-    if "auto" in (order_arg["primary"], order_arg["bounce"]) and sched == L.VRT_SCHED_GRID:
-        # the grid's block order decides which waves form each launch's tail; results are the
-        # same: time whole steps with each pair of orders and keep the fastest pair
+    auto_orders = "auto" in (order_arg["primary"], order_arg["bounce"]) and sched == L.VRT_SCHED_GRID
+    if auto_orders or args.bounce_generation == "auto":
+        # the grid's block order decides which waves form each launch's tail and the generation
+        # order which rays share a wave; results per ray are the same: time whole steps with each
+        # combination and keep the fastest
         both = (L.VRT_ORDER_FORWARD, L.VRT_ORDER_REVERSE)
-        for po in both if order_arg["primary"] == "auto" else (order["primary"],):
-            for bo in both if order_arg["bounce"] == "auto" else (order["bounce"],):
-                order["primary"], order["bounce"] = po, bo
-                cur["primary"], cur["bounce"] = opts(), opts(cnt=count, which="bounce")
-                trial[(po, bo)] = _time_ms(lambda: (cast_primary(), gen_bounce(), cast_bounce()), 20)
-        order["primary"], order["bounce"] = min(trial, key=trial.get)
+        gens = tuple(gen_orders.values()) if args.bounce_generation == "auto" else (cur["gen"],)
+        for go in gens:
+            for po in both if auto_orders and order_arg["primary"] == "auto" else (order["primary"],):
+                for bo in both if auto_orders and order_arg["bounce"] == "auto" else (order["bounce"],):
+                    order["primary"], order["bounce"], cur["gen"] = po, bo, go
+                    cur["primary"], cur["bounce"] = opts(), opts(cnt=count, which="bounce")
+                    trial[(go, po, bo)] = _time_ms(lambda: (cast_primary(), gen_bounce(), cast_bounce()), 20)
+        cur["gen"], order["primary"], order["bounce"] = min(trial, key=trial.get)
         cur["primary"], cur["bounce"] = opts(), opts(cnt=count, which="bounce")
     names = ("forward", "reverse")
+    gen_names = {v: k for k, v in gen_orders.items()}
 
     # counters (same traversal, counting kernel variant; outside the timed region)
     cp = torch.zeros(4, dtype=torch.int64, device=dev)
@@ -563,7 +573,8 @@ def c2_step(world, cam, args, K: int, order_arg: dict, event_every: int) -> dict
         "gen_ms": [ev[k][1].elapsed_time(ev[k][2]) for k in sampled],
         "bounce_ms": [ev[k][2].elapsed_time(ev[k][3]) for k in sampled],
         "order": {k: names[v] for k, v in order.items() if k != "fused"},
-        "order_trials": {f"{names[p]}/{names[b]}": round(t, 4) for (p, b), t in trial.items()},
+        "bounce_generation": gen_names[cur["gen"]],
+        "order_trials": {f"{gen_names[g]}: {names[p]}/{names[b]}": round(t, 4) for (g, p, b), t in trial.items()},
     }
     res["value"] = (n + nb) * K / wall / 1e6
     res["ms_per_step"] = wall * 1e3 / K
@@ -594,6 +605,9 @@ def main() -> int:
     for launch in ("primary", "bounce", "fused"):
         ap.add_argument(f"--{launch}-order", default="auto", choices=["auto", "forward", "reverse"],
                         help=f"grid block order of the {launch} launch; auto: timed after warm-up")
+    ap.add_argument("--bounce-generation", default="auto", choices=["auto", "primary", "direction"],
+                    help="bounce rays in primary order or grouped by direction octant (vrt_bounce_generate_ex); "
+                         "auto: timed after warm-up with the block orders")
     ap.add_argument("--anchor", default="sector", choices=["floor", "sector"],
                     help="integer world origin of the camera rays: the camera's sector corner (default; the "
                          "aligned-origin step) or floor(eye)")
@@ -674,7 +688,8 @@ def main() -> int:
     order_arg = {"primary": args.primary_order, "bounce": args.bounce_order}
     st = c2_step(world, cam, args, args.steps, order_arg, args.event_every)
     if st["order_trials"]:
-        log("block order (primary, bounce): " + ", ".join(f"{k} {v:.4f} ms" for k, v in st["order_trials"].items()))
+        log("bounce generation: block order (primary/bounce): " +
+            ", ".join(f"{k} {v:.4f} ms" for k, v in st["order_trials"].items()))
     n, nb, K = st["n"], st["nb"], st["K"]
     prim, hits_p, hits_b, bounce_rays, src = st["prim"], st["hits_p"], st["hits_b"], st["bounce_rays"], st["src"]
     cp_h, cb_h = st["cp"], st["cb"]
@@ -704,7 +719,8 @@ def main() -> int:
     del ca, cb
     roofline["achievable"] = round(achievable, 1)
     roofline["frac_of_achievable"] = round(achieved / achievable, 4)
-    want = {"tile": args.tile, "schedule": args.schedule, "order": st["order"], "anchor": args.anchor}
+    want = {"tile": args.tile, "schedule": args.schedule, "order": st["order"], "anchor": args.anchor,
+            "bounce_generation": st["bounce_generation"]}
     tr = read_profile_json("traffic_cast_c2.json", want)
     if tr:
         roofline["traffic"] = tr.get("hbm_bytes_per_launch")
@@ -741,6 +757,7 @@ def main() -> int:
             "primary_ms": round(statistics.mean(fst["prim_ms"]), 5),
             "bounce_ms": round(statistics.mean(fst["bounce_ms"]), 5),
             "order": fst["order"],
+            "bounce_generation": fst["bounce_generation"],
             "bounce_rays": fst["nb"],
         }
         log(f"floor(eye) anchor: {floor['value']} Mrays/s ({floor['ms_per_step']:.4f} ms per step)")
@@ -778,8 +795,11 @@ def main() -> int:
         ow = O.OracleWorld.from_stream(scene.stream())
         po = prim.o.cpu().numpy()
         pd = prim.d.cpu().numpy()
-        bo = bounce_rays.o[:, :nb].cpu().numpy()
-        bd = bounce_rays.d[:, :nb].cpu().numpy()
+        # the CPU path traces the bounce rays in primary order, as RenderRow does per pixel
+        # (CpuRenderer.cpp:389-392); the GPU's hits are compared in that order
+        perm = np.argsort(src[:nb].cpu().numpy().astype(np.int64), kind="stable")
+        bo = np.ascontiguousarray(bounce_rays.o[:, :nb].cpu().numpy()[:, perm])
+        bd = np.ascontiguousarray(bounce_rays.d[:, :nb].cpu().numpy()[:, perm])
         ow.cast_simd(po, pd, prim.world_origin, W.MAX_ITERS, threads=cpu_threads)  # warm-up
         times = []
         for _ in range(args.cpu_runs):
@@ -789,8 +809,8 @@ def main() -> int:
             times.append(time.perf_counter() - t0)
         med = statistics.median(times)
         # same rays, same answers: the GPU against the CPU path
-        ok = np.array_equal(hits_p.rec.cpu().numpy(), rp["rec"]) and np.array_equal(hits_b.rec[:nb].cpu().numpy(),
-                                                                                    rb["rec"])
+        ok = np.array_equal(hits_p.rec.cpu().numpy(), rp["rec"]) and np.array_equal(
+            hits_b.rec[:nb].cpu().numpy()[perm], rb["rec"])
         cpu = {
             "value": round((n + nb) / med / 1e6, 2),
             "unit": "Mrays/s",
@@ -836,6 +856,7 @@ def main() -> int:
             "tile": args.tile,
             "schedule": args.schedule,
             "order": want["order"],
+            "bounce_generation": want["bounce_generation"],
             "camera_anchor": f"{args.anchor}: world origin {list(prim.world_origin)}, eye {[round(x, 4) for x in cam.c.eye]}",
             "parallelism": "single GPU",
         },

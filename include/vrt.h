@@ -334,6 +334,29 @@ int vrt_bounce_generate(const vrt_rays* primary, const vrt_hits* hits, uint32_t 
                         void* scratch, void* stream);
 
 /*
+ * vrt_bounce_generate with a choice of output order (the rays, src entries and count are the
+ * same; only their order differs):
+ *   VRT_BOUNCE_PRIMARY_ORDER    increasing primary index (vrt_bounce_generate);
+ *   VRT_BOUNCE_DIRECTION_ORDER  grouped by the direction's octant, key 0..7 (bit a set when
+ *                               component a of N + s is >= 0, read off the signs of its terms:
+ *                               the normal's on its axis, elsewhere s's from its two PCG
+ *                               samples; the octant of d except at sign boundaries), in
+ *                               increasing primary index within a key.  Neighbouring bounce
+ *                               rays then share a direction octant as well as a region, which
+ *                               makes a following cast of them faster.  At most 8388608
+ *                               primaries.
+ * No counterpart in the reference (its bounce rays are traced per pixel).
+ */
+typedef enum vrt_bounce_order {
+    VRT_BOUNCE_PRIMARY_ORDER = 0,
+    VRT_BOUNCE_DIRECTION_ORDER = 1
+} vrt_bounce_order;
+
+int vrt_bounce_generate_ex(const vrt_rays* primary, const vrt_hits* hits, uint32_t width,
+                           uint32_t seed, uint32_t order, vrt_ray_buffers* out, uint32_t* src,
+                           uint32_t* count, void* scratch, void* stream);
+
+/*
  * vrt_bounce_generate without a workspace, and allocating nothing: no counterpart in the
  * reference.  For callers that cannot allocate device memory: code under HIP graph capture,
  * tight serving loops, harnesses that forbid hipMalloc.  The per-block hit counts live in a

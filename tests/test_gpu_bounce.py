@@ -196,3 +196,84 @@ def test_bounce_generation_paths_match_host_twin(vrt, cuda, n, frac):
     assert np.array_equal(src[:nb].cpu().numpy().astype(np.uint32), hsrc[:nb])
     assert np.array_equal(out[:3, :nb].view(np.uint32), br.o[:, :nb].cpu().numpy().view(np.uint32))
     assert np.array_equal(out[3:, :nb].view(np.uint32), br.d[:, :nb].cpu().numpy().view(np.uint32))
+
+
+def _bounce_keys(info, src, width, seed):
+    """Host model of the direction-order key: the PCG samples (RandomGen.glsl:6-20) of primary src
+    and the signs of N + s's terms (s = (sin 2 pi u2 sy, 2 u1 - 1, cos 2 pi u2 sy))."""
+    M = np.uint32
+    px, py = (src % width).astype(M), (src // width).astype(M)
+    with np.errstate(over="ignore"):
+        st = ((M(1103515245) * px) ^ ((M(1103515245) * py) >> M(3))) + M(seed) * M(12345)
+
+        def nxt(s):
+            word = ((s >> ((s >> M(28)) + M(4))) ^ s) * M(277803737)
+            return s * M(747796405) + M(2891336453), (word >> M(22)) ^ word
+
+        st, w1 = nxt(st)
+        st, w2 = nxt(st)
+    u1 = w1.astype(np.float32) * np.float32(2.0 ** -32)
+    u2 = w2.astype(np.float32) * np.float32(2.0 ** -32)
+    info = info.astype(np.uint32)
+    cx, cy, cz = (info >> 8) & 3, (info >> 10) & 3, (info >> 12) & 3
+    bx = np.where(cx != 0, cx == 1, u2 < 0.5)
+    by = np.where(cy != 0, cy == 1, u1 >= 0.5)
+    bz = np.where(cz != 0, cz == 1, (u2 < 0.25) | (u2 >= 0.75))
+    return bx.astype(np.int64) | by.astype(np.int64) << 1 | bz.astype(np.int64) << 2
+
+
+@pytest.mark.parametrize("config", ["c1", "c2"])
+def test_direction_order_generation(vrt, cuda, config, c2_scene):
+    """vrt_bounce_generate_ex in direction order writes the primary-order set's rays, sources and
+    count, permuted: grouped by key (the host model of the key), increasing primary index within
+    a key; the key is the octant of the written direction for all but sign-boundary rays; the
+    cast of either order gives each ray the same hit."""
+    import voxelrt_amd as V
+    from voxelrt_amd import workloads as W
+
+    scene, cam = (W.C1_WORLD.build(), W.c1_camera()) if config == "c1" else (c2_scene, W.c2_camera())
+    world = V.World.from_scene(scene)
+    prim = cam.rays_device()
+    hits = world.cast(prim, W.MAX_ITERS, with_pos=True)
+    b0, s0, c0 = V.bounce(prim, hits, cam.c.width, W.BOUNCE_SEED)
+    b1, s1, c1 = V.bounce(prim, hits, cam.c.width, W.BOUNCE_SEED, order=1)
+    torch.cuda.synchronize()
+    nb = int(c0.item())
+    assert int(c1.item()) == nb and nb > 0
+    src0, src1 = s0[:nb].cpu().numpy().astype(np.int64), s1[:nb].cpu().numpy().astype(np.int64)
+    assert np.all(np.diff(src0) > 0)
+    j = np.searchsorted(src0, src1)
+    assert np.array_equal(src0[j], src1) and len(np.unique(src1)) == nb
+    for a in ("o", "d"):
+        g0, g1 = getattr(b0, a)[:, :nb].cpu().numpy(), getattr(b1, a)[:, :nb].cpu().numpy()
+        assert np.array_equal(g1.view(np.uint32), g0[:, j].view(np.uint32)), a
+    info = hits.rec[:, 3].cpu().numpy()[src1]
+    key = _bounce_keys(info, src1, cam.c.width, W.BOUNCE_SEED)
+    assert np.all(np.diff(key) >= 0)
+    same_key = np.diff(key) == 0
+    assert np.all(np.diff(src1)[same_key] > 0)
+    d1 = b1.d[:, :nb].cpu().numpy()
+    octant = (d1[0] >= 0).astype(np.int64) | (d1[1] >= 0).astype(np.int64) << 1 | (d1[2] >= 0).astype(np.int64) << 2
+    assert (octant == key).mean() > 0.9999
+    h0 = world.cast(b0, W.MAX_ITERS, count=c0)
+    h1 = world.cast(b1, W.MAX_ITERS, count=c1)
+    torch.cuda.synchronize()
+    r0, r1 = h0.rec[:nb].cpu().numpy(), h1.rec[:nb].cpu().numpy()
+    assert np.array_equal(r1, r0[j])
+    assert np.array_equal(h1.t[:nb].cpu().numpy().view(np.uint32), h0.t[:nb].cpu().numpy()[j].view(np.uint32))
+
+
+def test_direction_order_limits(vrt, cuda):
+    """Direction order takes at most 8388608 primaries (rejected before any launch), and no
+    primaries writes a zero count."""
+    import voxelrt_amd as V
+
+    empty = V.RayBatch.empty(0)
+    br, src, cnt = V.bounce(empty, V.HitBatch.empty(0, with_pos=True), 16, 1, order=1)
+    torch.cuda.synchronize()
+    assert int(cnt.item()) == 0
+    with pytest.raises(V.VrtError, match="unknown order"):
+        V.bounce(empty, V.HitBatch.empty(0, with_pos=True), 16, 1, order=2)
+    n = 8388609
+    with pytest.raises(V.VrtError, match="at most 8388608"):
+        V.bounce(V.RayBatch.empty(n), V.HitBatch.empty(n, with_pos=True), 4096, 1, order=1)

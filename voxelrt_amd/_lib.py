@@ -22,6 +22,9 @@ VRT_SCHED_GRID = 1
 VRT_SCHED_PERSISTENT = 2
 VRT_SCHED_COMPACT = 3
 
+VRT_BOUNCE_PRIMARY_ORDER = 0
+VRT_BOUNCE_DIRECTION_ORDER = 1
+
 VRT_ORDER_FORWARD = 0
 VRT_ORDER_REVERSE = 1
 
@@ -175,6 +178,7 @@ _SIGNATURES = {
     "vrt_cast_one_f64": (C.c_int, [P, C.POINTER(C.c_double * 3), C.POINTER(C.c_double * 3), u32, C.POINTER(HitF64)]),
     "vrt_bounce_scratch_bytes": (C.c_size_t, [u32]),
     "vrt_bounce_generate": (C.c_int, [C.POINTER(Rays), C.POINTER(Hits), u32, u32, C.POINTER(RayBuffers), P, P, P, P]),
+    "vrt_bounce_generate_ex": (C.c_int, [C.POINTER(Rays), C.POINTER(Hits), u32, u32, u32, C.POINTER(RayBuffers), P, P, P, P]),
     "vrt_bounce_generate_noalloc": (C.c_int, [C.POINTER(Rays), C.POINTER(Hits), u32, u32, C.POINTER(RayBuffers), P, P,
                                               P]),
     "vrt_camera_rays": (C.c_int, [C.POINTER(Camera), C.POINTER(RayBuffers), P]),

@@ -484,8 +484,9 @@ def new_counters(device="cuda") -> torch.Tensor:
     return torch.zeros(4, dtype=torch.int64, device=device)
 
 
-def bounce(rays: RayBatch, hits: HitBatch, width: int, seed: int, stream=None):
-    """Diffuse bounce rays of the hits (vrt_bounce_generate); returns (RayBatch, src, count tensor)."""
+def bounce(rays: RayBatch, hits: HitBatch, width: int, seed: int, stream=None, order: int = 0):
+    """Diffuse bounce rays of the hits (vrt_bounce_generate, or vrt_bounce_generate_ex for order 1:
+    grouped by direction octant); returns (RayBatch, src, count tensor)."""
     assert hits.pos is not None, "bounce generation needs hit positions"
     n = rays.n
     dev = rays.o.device
@@ -496,8 +497,13 @@ def bounce(rays: RayBatch, hits: HitBatch, width: int, seed: int, stream=None):
     r = rays.to_c()
     h = hits.to_c()
     bufs = out.buffers()
-    L.check(L.lib().vrt_bounce_generate(C.byref(r), C.byref(h), width, seed, C.byref(bufs), src.data_ptr(),
-                                        count.data_ptr(), scratch.data_ptr(), _stream_handle(stream)))
+    if order:
+        L.check(L.lib().vrt_bounce_generate_ex(C.byref(r), C.byref(h), width, seed, order, C.byref(bufs),
+                                               src.data_ptr(), count.data_ptr(), scratch.data_ptr(),
+                                               _stream_handle(stream)))
+    else:
+        L.check(L.lib().vrt_bounce_generate(C.byref(r), C.byref(h), width, seed, C.byref(bufs), src.data_ptr(),
+                                            count.data_ptr(), scratch.data_ptr(), _stream_handle(stream)))
     return out, src, count
 
 

@@ -9,6 +9,7 @@
 // primaries count -> scan -> scatter; deterministic, no atomics).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -235,6 +236,195 @@ __global__ __launch_bounds__(1024) void bounce_write_rows_kernel(const int4* __r
     }
 }
 
+// ---------------------------------------------------------------- direction order
+
+// Group key of hit i's bounce ray: bit a set when component a of N + s is >= 0, read off the
+// terms' signs (d = normalize(N + s), cosine_dir): on the normal's axis the normal's sign,
+// elsewhere the sign of s's component from the two PCG samples cosine_dir draws,
+// s = (sin(2 pi u2) sy, 2 u1 - 1, cos(2 pi u2) sy) with sy >= 0.  It is d's octant except
+// for rays with a component at a sign boundary; both passes compute it with this function,
+// so the placement never depends on rounding.  8: no hit, no bounce ray.  u1, u2: the samples.
+__device__ __forceinline__ uint32_t bounce_key(uint32_t info, uint32_t i, uint32_t width, uint32_t seed, float& u1,
+                                               float& u2) {
+    if ((info & 0xFFu) == 0) return 8u;
+    uint32_t st = pcg_init(i % width, i / width, seed);
+    u1 = pcg_unit(st);
+    u2 = pcg_unit(st);
+    const uint32_t cx = (info >> 8) & 3u, cy = (info >> 10) & 3u, cz = (info >> 12) & 3u;  // 1: +, 3: -
+    const bool bx = cx ? cx == 1u : u2 < 0.5f;
+    const bool by = cy ? cy == 1u : u1 >= 0.5f;
+    const bool bz = cz ? cz == 1u : (u2 < 0.25f || u2 >= 0.75f);
+    return (bx ? 1u : 0u) | (by ? 2u : 0u) | (bz ? 4u : 0u);
+}
+
+// The lanes of the wave whose key equals this lane's, from one ballot per key bit (keys are
+// 0..8; lanes with key 8 are left out of every mask); pops[o]: the count of key o (o < 8),
+// the same for every lane.
+__device__ __forceinline__ uint64_t same_key_lanes(uint32_t key, uint32_t pops[8]) {
+    const uint64_t valid = __ballot(key < 8u);
+    const uint64_t b0 = __ballot(key & 1u), b1 = __ballot(key & 2u), b2 = __ballot(key & 4u);
+#pragma unroll
+    for (uint32_t o = 0; o < 8; o++)
+        pops[o] = (uint32_t)__popcll(valid & (o & 1u ? b0 : ~b0) & (o & 2u ? b1 : ~b1) & (o & 4u ? b2 : ~b2));
+    return valid & (key & 1u ? b0 : ~b0) & (key & 2u ? b1 : ~b1) & (key & 4u ? b2 : ~b2);
+}
+
+// Per block of kGenBlock primaries, the hits of each key: counts[key * nblocks + block].
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void bounce_count_keys_kernel(const int4* __restrict__ rec, uint32_t n,
+                                                                 uint32_t width, uint32_t seed, uint32_t nblocks,
+                                                                 uint32_t* __restrict__ counts) {
+    __shared__ uint32_t wave_counts[16][8];
+    const uint32_t t = threadIdx.x, b = blockIdx.x;
+    uint32_t c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (uint32_t j = 0; j < kGenRows; j++) {
+        const uint32_t i = b * kGenBlock + j * 1024 + t;
+        float u1, u2;
+        const uint32_t key = i < n ? bounce_key((uint32_t)rec[i].w, i, width, seed, u1, u2) : 8u;
+        uint32_t pops[8];
+        (void)same_key_lanes(key, pops);
+#pragma unroll
+        for (uint32_t o = 0; o < 8; o++) c[o] += pops[o];
+    }
+    if (__lane_id() == 0)
+#pragma unroll
+        for (uint32_t o = 0; o < 8; o++) wave_counts[t >> 6][o] = c[o];
+    __syncthreads();
+    if (t < 8) {
+        uint32_t v = 0;
+        for (uint32_t w = 0; w < 16; w++) v += wave_counts[w][t];
+        counts[t * nblocks + b] = v;
+    }
+}
+
+// Writes block b's bounce rays: key o's rays go after those of keys < o (all blocks) and of key
+// o in the blocks before b, in primary order within the block.  Block 0 writes the total.
+// Each row of 1024 primaries is staged in LDS sorted by key and copied out in 8 contiguous
+// runs, one per key, so the global stores coalesce (written straight from the lanes, a wave's
+// stores would split into a run per key).
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void bounce_write_keys_kernel(const int4* __restrict__ rec,
+                                                                 const float* __restrict__ px,
+                                                                 const float* __restrict__ py,
+                                                                 const float* __restrict__ pz, uint32_t n,
+                                                                 uint32_t width, uint32_t seed,
+                                                                 const uint32_t* __restrict__ counts,
+                                                                 uint32_t nblocks, vrt_ray_buffers out,
+                                                                 uint32_t* __restrict__ src,
+                                                                 uint32_t* __restrict__ count) {
+    __shared__ uint32_t key_before[8], key_total[8], key_base[8];
+    __shared__ uint32_t local[8 * kGenRows * 16];  // per (key, row, wave): rays, then their exclusive prefix
+    __shared__ uint32_t row_start[kGenRows][9];    // per row: where key o's rays start in the row's sorted list
+    __shared__ float stage[6][1024];               // one row's rays sorted by key: o, d
+    __shared__ uint32_t stage_src[1024];           // and their primary indices
+    const uint32_t t = threadIdx.x, lane = __lane_id(), wave = t >> 6, b = blockIdx.x;
+    // key `wave` (waves 0..7): its rays in the blocks before b and in all blocks
+    if (wave < 8) {
+        const uint32_t* c = counts + wave * nblocks;
+        uint32_t bef = 0, tot = 0;
+        for (uint32_t k = lane; k < nblocks; k += 64) {  // kGenMaxBlocks / 64 loads at most
+            const uint32_t v = c[k];
+            tot += v;
+            bef += k < b ? v : 0u;
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            bef += __shfl_xor(bef, off, 64);
+            tot += __shfl_xor(tot, off, 64);
+        }
+        if (lane == 0) {
+            key_before[wave] = bef;
+            key_total[wave] = tot;
+        }
+    }
+    uint32_t info[kGenRows], key[kGenRows];
+    float u1[kGenRows], u2[kGenRows];
+    uint64_t same[kGenRows];  // the lanes of this wave and row with this lane's key
+#pragma unroll
+    for (uint32_t j = 0; j < kGenRows; j++) {
+        const uint32_t i = b * kGenBlock + j * 1024 + t;
+        info[j] = i < n ? (uint32_t)rec[i].w : 0u;
+        key[j] = i < n ? bounce_key(info[j], i, width, seed, u1[j], u2[j]) : 8u;
+        uint32_t pops[8];
+        same[j] = same_key_lanes(key[j], pops);
+        if (lane == 0)
+#pragma unroll
+            for (uint32_t o = 0; o < 8; o++) local[(o * kGenRows + j) * 16 + wave] = pops[o];
+    }
+    __syncthreads();
+    if (t < 8) {
+        uint32_t v = key_before[t];
+        for (uint32_t o = 0; o < t; o++) v += key_total[o];
+        key_base[t] = v;
+    }
+    if (b == 0 && t == 0) {
+        uint32_t v = 0;
+        for (uint32_t o = 0; o < 8; o++) v += key_total[o];
+        *count = v;
+    }
+    // per row, where each key's rays start in the row's sorted list (rows' totals per key)
+    if (t < kGenRows) {
+        uint32_t v = 0;
+        for (uint32_t o = 0; o < 8; o++) {
+            row_start[t][o] = v;
+            for (uint32_t w = 0; w < 16; w++) v += local[(o * kGenRows + t) * 16 + w];
+        }
+        row_start[t][8] = v;
+    }
+    __syncthreads();
+    // exclusive scan of the 512 local counts in (key, row, wave) order within each key: wave o
+    // scans key o's 64 entries (row-major, so primary order)
+    if (t < 8 * kGenRows * 16) {
+        const uint32_t c = local[t];
+        uint32_t incl = c;
+#pragma unroll
+        for (uint32_t off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += y;
+        }
+        local[t] = incl - c;
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t j = 0; j < kGenRows; j++) {
+        const uint32_t i = b * kGenBlock + j * 1024 + t;
+        if (key[j] < 8u) {
+            const uint32_t kj = key[j];
+            // this ray's place in the row's sorted list: the row's keys before kj, this key's rays
+            // of the earlier waves of the row, then the lanes before this one
+            const uint32_t slot = row_start[j][kj] + local[(kj * kGenRows + j) * 16 + wave] -
+                                  local[(kj * kGenRows + j) * 16] +
+                                  __builtin_amdgcn_mbcnt_hi((uint32_t)(same[j] >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo((uint32_t)same[j], 0u));
+            const float p[3] = {px[i], py[i], pz[i]};
+            float o[3], d[3];
+            bounce_ray_samples(info[j], p, u1[j], u2[j], o, d);
+            stage[0][slot] = o[0];
+            stage[1][slot] = o[1];
+            stage[2][slot] = o[2];
+            stage[3][slot] = d[0];
+            stage[4][slot] = d[1];
+            stage[5][slot] = d[2];
+            stage_src[slot] = i;
+        }
+        __syncthreads();
+        if (t < row_start[j][8]) {
+            // the key of sorted slot t, and where its run of this row goes
+            uint32_t kt = 0;
+#pragma unroll
+            for (uint32_t o = 1; o < 8; o++) kt += t >= row_start[j][o] ? 1u : 0u;
+            const uint32_t k = key_base[kt] + local[(kt * kGenRows + j) * 16] + (t - row_start[j][kt]);
+            out.ox[k] = stage[0][t];
+            out.oy[k] = stage[1][t];
+            out.oz[k] = stage[2][t];
+            out.dx[k] = stage[3][t];
+            out.dy[k] = stage[4][t];
+            out.dz[k] = stage[5][t];
+            src[k] = stage_src[t];
+        }
+        __syncthreads();
+    }
+}
+
 }  // namespace vrt
 
 using namespace vrt;
@@ -277,7 +467,7 @@ static int scratch_of_current_device(uint32_t** out) {
 // the two-launch range).  Every block count the launches read is written by them first.
 static int bounce_generate(const vrt_rays* primary, const vrt_hits* hits, uint32_t width, uint32_t seed,
                            vrt_ray_buffers* out, uint32_t* src, uint32_t* count, uint32_t* blocks, uint32_t max_n,
-                           hipStream_t s, const char* entry) {
+                           hipStream_t s, const char* entry, uint32_t order = VRT_BOUNCE_PRIMARY_ORDER) {
     const std::string name(entry);
     if (!primary || !hits || !out || !count) return fail(VRT_ERR_INVALID_ARGUMENT, name + ": null argument");
     if (width == 0) return fail(VRT_ERR_INVALID_ARGUMENT, name + ": width must be > 0");
@@ -291,6 +481,9 @@ static int bounce_generate(const vrt_rays* primary, const vrt_hits* hits, uint32
     if (n && !hits->pos) return fail(VRT_ERR_INVALID_ARGUMENT, name + ": hits need positions");
     if (n && (!out->ox || !out->oy || !out->oz || !out->dx || !out->dy || !out->dz))
         return fail(VRT_ERR_INVALID_ARGUMENT, name + ": null ray buffer");
+    if (order > VRT_BOUNCE_DIRECTION_ORDER) return fail(VRT_ERR_INVALID_ARGUMENT, name + ": unknown order");
+    if (order == VRT_BOUNCE_DIRECTION_ORDER && n > kGenMaxBlocks * kGenBlock)
+        return fail(VRT_ERR_INVALID_ARGUMENT, name + ": direction order takes at most 8388608 primary rays");
     if (n == 0) {
         // writes *count = 0 and reads no block count
         hipLaunchKernelGGL(bounce_scan_kernel, dim3(1), dim3(1024), 0, s, blocks, 0u, count);
@@ -299,6 +492,13 @@ static int bounce_generate(const vrt_rays* primary, const vrt_hits* hits, uint32
     }
     const int4* rec = (const int4*)hits->rec;
     const uint32_t nrows = (n + kGenBlock - 1) / kGenBlock;
+    if (order == VRT_BOUNCE_DIRECTION_ORDER) {  // nrows <= kGenMaxBlocks; blocks holds 8 * nrows counts
+        bounce_count_keys_kernel<<<nrows, 1024, 0, s>>>(rec, n, width, seed, nrows, blocks);
+        bounce_write_keys_kernel<<<nrows, 1024, 0, s>>>(rec, hits->pos, hits->pos + n, hits->pos + 2 * (size_t)n, n,
+                                                        width, seed, blocks, nrows, *out, src, count);
+        VRT_HIP(hipGetLastError());
+        return VRT_OK;
+    }
     if (nrows <= kGenMaxBlocks) {
         bounce_count_rows_kernel<<<nrows, 1024, 0, s>>>(rec, n, blocks);
         bounce_write_rows_kernel<<<nrows, 1024, 0, s>>>(rec, hits->pos, hits->pos + n, hits->pos + 2 * (size_t)n, n,
@@ -338,7 +538,10 @@ int vrt_camera_rays(const vrt_camera* cam, vrt_ray_buffers* out, void* stream) {
 }
 
 size_t vrt_bounce_scratch_bytes(uint32_t n) {
-    return (size_t)(((uint64_t)n + kBounceBlock - 1) / kBounceBlock + 1) * sizeof(uint32_t);
+    // one count per block of kBounceBlock primaries (+1), or 8 per block of kGenBlock (direction order)
+    const uint64_t per_block = ((uint64_t)n + kBounceBlock - 1) / kBounceBlock + 1;
+    const uint64_t per_key = 8 * (((uint64_t)n + kGenBlock - 1) / kGenBlock);
+    return (size_t)std::max(per_block, per_key) * sizeof(uint32_t);
 }
 
 int vrt_bounce_generate(const vrt_rays* primary, const vrt_hits* hits, uint32_t width, uint32_t seed,
@@ -348,6 +551,17 @@ int vrt_bounce_generate(const vrt_rays* primary, const vrt_hits* hits, uint32_t 
             return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_bounce_generate: null argument");
         return bounce_generate(primary, hits, width, seed, out, src, count, (uint32_t*)workspace, kMaxBouncePrimaries,
                                (hipStream_t)stream, "vrt_bounce_generate");
+    });
+}
+
+int vrt_bounce_generate_ex(const vrt_rays* primary, const vrt_hits* hits, uint32_t width, uint32_t seed,
+                           uint32_t order, vrt_ray_buffers* out, uint32_t* src, uint32_t* count, void* workspace,
+                           void* stream) {
+    return guarded([&]() -> int {
+        if (primary && primary->n && !workspace)
+            return fail(VRT_ERR_INVALID_ARGUMENT, "vrt_bounce_generate_ex: null argument");
+        return bounce_generate(primary, hits, width, seed, out, src, count, (uint32_t*)workspace, kMaxBouncePrimaries,
+                               (hipStream_t)stream, "vrt_bounce_generate_ex", order);
     });
 }
 

@@ -100,9 +100,7 @@ __host__ __device__ inline void tile_pixel(uint32_t i, uint32_t width, uint32_t 
 // normalize(N + s) for s uniform on the unit sphere from the next two PCG
 // samples of st (random_dir, RandomGen.glsl:37-48, with PCG samples in place of
 // the blue-noise texture); N + s = 0 falls back to N.
-__host__ __device__ inline void cosine_dir(const float N[3], uint32_t& st, float dd[3]) {
-    float u1 = pcg_unit(st);
-    float u2 = pcg_unit(st);
+__host__ __device__ inline void cosine_dir_samples(const float N[3], float u1, float u2, float dd[3]) {
     float y = u1 * 2.0f - 1.0f;
     float sn, cs;
     sincos_turns(u2, sn, cs);
@@ -116,20 +114,33 @@ __host__ __device__ inline void cosine_dir(const float N[3], uint32_t& st, float
         dd[0] = v0 * il; dd[1] = v1 * il; dd[2] = v2 * il;
     }
 }
+__host__ __device__ inline void cosine_dir(const float N[3], uint32_t& st, float dd[3]) {
+    float u1 = pcg_unit(st);
+    float u2 = pcg_unit(st);
+    cosine_dir_samples(N, u1, u2, dd);
+}
 
 // Diffuse bounce of one hit (CpuRenderer.cpp:389-392, VoxelRender.comp:63-71):
 // origin = pos + N * 0.01, direction = normalize(N + s), s uniform on the unit
 // sphere from two PCG samples; N + s = 0 falls back to N.
-__host__ __device__ inline void bounce_ray(uint32_t info, const float pos[3], uint32_t i, uint32_t width,
-                                           uint32_t seed, float o[3], float d[3]) {
+// The same from hit i's two PCG samples (u1, u2: the first two of pcg_init(i % width, i / width,
+// seed)), for callers that already drew them.
+__host__ __device__ inline void bounce_ray_samples(uint32_t info, const float pos[3], float u1, float u2, float o[3],
+                                                   float d[3]) {
     float N[3] = {normal_component(info, 0), normal_component(info, 1), normal_component(info, 2)};
-    uint32_t st = pcg_init(i % width, i / width, seed);
     float dd[3];
-    cosine_dir(N, st, dd);
+    cosine_dir_samples(N, u1, u2, dd);
     for (int a = 0; a < 3; a++) {
         o[a] = pos[a] + N[a] * 0.01f;
         d[a] = nudge_zero(dd[a]);
     }
+}
+__host__ __device__ inline void bounce_ray(uint32_t info, const float pos[3], uint32_t i, uint32_t width,
+                                           uint32_t seed, float o[3], float d[3]) {
+    uint32_t st = pcg_init(i % width, i / width, seed);
+    float u1 = pcg_unit(st);
+    float u2 = pcg_unit(st);
+    bounce_ray_samples(info, pos, u1, u2, o, d);
 }
 
 // IEEE binary16 bits -> float (exact).
