@@ -196,6 +196,18 @@ def test_bounce_generation_paths_match_host_twin(vrt, cuda, n, frac):
     assert np.array_equal(src[:nb].cpu().numpy().astype(np.uint32), hsrc[:nb])
     assert np.array_equal(out[:3, :nb].view(np.uint32), br.o[:, :nb].cpu().numpy().view(np.uint32))
     assert np.array_equal(out[3:, :nb].view(np.uint32), br.d[:, :nb].cpu().numpy().view(np.uint32))
+    if n <= 8388608:
+        # direction order: the same rays, grouped by key, primary order within a key
+        b1, s1, c1 = V.bounce(prim, hits, 1920, 7, order=1)
+        torch.cuda.synchronize()
+        assert int(c1.item()) == nb
+        src1 = s1[:nb].cpu().numpy().astype(np.int64)
+        j = np.searchsorted(hsrc[:nb].astype(np.int64), src1)
+        assert np.array_equal(hsrc[:nb][j].astype(np.int64), src1)
+        assert np.array_equal(b1.d[:, :nb].cpu().numpy().view(np.uint32), out[3:, :nb][:, j].view(np.uint32))
+        assert np.array_equal(b1.o[:, :nb].cpu().numpy().view(np.uint32), out[:3, :nb][:, j].view(np.uint32))
+        key = _bounce_keys(rec[src1, 3], src1, 1920, 7)
+        assert np.all(np.diff(key) >= 0) and np.all(np.diff(src1)[np.diff(key) == 0] > 0)
 
 
 def _bounce_keys(info, src, width, seed):
