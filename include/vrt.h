@@ -28,12 +28,15 @@
  *                          (src/VoxelRT/VoxelMap.h:213-214)
  *   vrt_bounce_generate    diffuse bounce construction in RenderRow
  *                          (src/VoxelRT/CpuRenderer.cpp:342-392,
- *                           src/VoxelRT/Shaders/VoxelRender.comp:63-86)
+ *                           src/VoxelRT/Shaders/VoxelRender.comp:63-86);
+ *                          vrt_bounce_generate_ex also groups the rays by direction
  *   vrt_cast_shader        rayCast / rayCastCoarse of the GPU renderer
  *                          (src/VoxelRT/Shaders/VoxelTraversal.glsl:92-243)
  *   vrt_render_frame       GpuRenderer::RenderFrame's trace and shading
  *                          (src/VoxelRT/GpuRenderer.cpp:239-273,
  *                           src/VoxelRT/Shaders/VoxelRender.comp:28-94)
+ * vrt_stream_release frees the per-stream state of the persistent and compact cast
+ * schedules; it has no counterpart in the reference.
  * The C++ interface of the reference (HitResult, VoxelMap::RayCast, a Renderer's
  * RenderFrame) sits over these calls in vrt.hpp; world files and seeded worlds
  * are in vrt_scene.h.
