@@ -299,9 +299,9 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void 
 
 // Writes block b's bounce rays: key o's rays go after those of keys < o (all blocks) and of key
 // o in the blocks before b, in primary order within the block.  Block 0 writes the total.
-// Each row of 1024 primaries is staged in LDS sorted by key and copied out in 8 contiguous
-// runs, one per key, so the global stores coalesce (written straight from the lanes, a wave's
-// stores would split into a run per key).
+// Rows of 1024 primaries are staged in LDS two at a time, sorted by key, and copied out in 8
+// contiguous runs, one per key, so the global stores coalesce (written straight from the
+// lanes, a wave's stores would split into a run per key).
 __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void bounce_write_keys_kernel(const int4* __restrict__ rec,
                                                                  const float* __restrict__ px,
                                                                  const float* __restrict__ py,
@@ -313,9 +313,12 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void 
                                                                  uint32_t* __restrict__ count) {
     __shared__ uint32_t key_before[8], key_total[8], key_base[8];
     __shared__ uint32_t local[8 * kGenRows * 16];  // per (key, row, wave): rays, then their exclusive prefix
-    __shared__ uint32_t row_start[kGenRows][9];    // per row: where key o's rays start in the row's sorted list
-    __shared__ float stage[6][1024];               // one row's rays sorted by key: o, d
-    __shared__ uint32_t stage_src[1024];           // and their primary indices
+    // rows are staged kStageRows at a time (a staging round)
+    constexpr uint32_t kStageRows = 2, kRounds = kGenRows / kStageRows, kStage = kStageRows * 1024;
+    __shared__ uint32_t round_start[kRounds][9];   // per round: where key o's rays start in its sorted list
+    __shared__ uint32_t key_block[8];              // key o's rays in this block
+    __shared__ float stage[6][kStage];             // one round's rays sorted by key: o, d
+    __shared__ uint32_t stage_src[kStage];         // and their primary indices
     const uint32_t t = threadIdx.x, lane = __lane_id(), wave = t >> 6, b = blockIdx.x;
     // key `wave` (waves 0..7): its rays in the blocks before b and in all blocks
     if (wave < 8) {
@@ -361,18 +364,10 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void 
         for (uint32_t o = 0; o < 8; o++) v += key_total[o];
         *count = v;
     }
-    // per row, where each key's rays start in the row's sorted list (rows' totals per key)
-    if (t < kGenRows) {
-        uint32_t v = 0;
-        for (uint32_t o = 0; o < 8; o++) {
-            row_start[t][o] = v;
-            for (uint32_t w = 0; w < 16; w++) v += local[(o * kGenRows + t) * 16 + w];
-        }
-        row_start[t][8] = v;
-    }
     __syncthreads();
     // exclusive scan of the 512 local counts in (key, row, wave) order within each key: wave o
-    // scans key o's 64 entries (row-major, so primary order)
+    // scans key o's 64 entries (row-major, so primary order); its last lane holds the block's
+    // count of key o
     if (t < 8 * kGenRows * 16) {
         const uint32_t c = local[t];
         uint32_t incl = c;
@@ -382,44 +377,65 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void 
             if (lane >= off) incl += y;
         }
         local[t] = incl - c;
+        if (lane == 63) key_block[wave] = incl;
+    }
+    __syncthreads();
+    // per staging round, where each key's rays start in the round's sorted list: key o has
+    // (prefix at the round's end) - (prefix at its start) rays in the round
+    if (t < kRounds) {
+        uint32_t v = 0;
+        for (uint32_t o = 0; o < 8; o++) {
+            round_start[t][o] = v;
+            const uint32_t first = local[(o * kGenRows + t * kStageRows) * 16];
+            const uint32_t end = t + 1 < kRounds ? local[(o * kGenRows + (t + 1) * kStageRows) * 16] : key_block[o];
+            v += end - first;
+        }
+        round_start[t][8] = v;
     }
     __syncthreads();
 #pragma unroll
-    for (uint32_t j = 0; j < kGenRows; j++) {
-        const uint32_t i = b * kGenBlock + j * 1024 + t;
-        if (key[j] < 8u) {
-            const uint32_t kj = key[j];
-            // this ray's place in the row's sorted list: the row's keys before kj, this key's rays
-            // of the earlier waves of the row, then the lanes before this one
-            const uint32_t slot = row_start[j][kj] + local[(kj * kGenRows + j) * 16 + wave] -
-                                  local[(kj * kGenRows + j) * 16] +
-                                  __builtin_amdgcn_mbcnt_hi((uint32_t)(same[j] >> 32),
-                                                            __builtin_amdgcn_mbcnt_lo((uint32_t)same[j], 0u));
-            const float p[3] = {px[i], py[i], pz[i]};
-            float o[3], d[3];
-            bounce_ray_samples(info[j], p, u1[j], u2[j], o, d);
-            stage[0][slot] = o[0];
-            stage[1][slot] = o[1];
-            stage[2][slot] = o[2];
-            stage[3][slot] = d[0];
-            stage[4][slot] = d[1];
-            stage[5][slot] = d[2];
-            stage_src[slot] = i;
+    for (uint32_t q = 0; q < kRounds; q++) {
+#pragma unroll
+        for (uint32_t r = 0; r < kStageRows; r++) {
+            const uint32_t j = q * kStageRows + r;
+            const uint32_t i = b * kGenBlock + j * 1024 + t;
+            if (key[j] < 8u) {
+                const uint32_t kj = key[j];
+                // this ray's place in the round's sorted list: the round's keys before kj, this
+                // key's rays of the round's earlier rows and waves, then the lanes before this one
+                const uint32_t slot = round_start[q][kj] + local[(kj * kGenRows + j) * 16 + wave] -
+                                      local[(kj * kGenRows + q * kStageRows) * 16] +
+                                      __builtin_amdgcn_mbcnt_hi((uint32_t)(same[j] >> 32),
+                                                                __builtin_amdgcn_mbcnt_lo((uint32_t)same[j], 0u));
+                const float p[3] = {px[i], py[i], pz[i]};
+                float o[3], d[3];
+                bounce_ray_samples(info[j], p, u1[j], u2[j], o, d);
+                stage[0][slot] = o[0];
+                stage[1][slot] = o[1];
+                stage[2][slot] = o[2];
+                stage[3][slot] = d[0];
+                stage[4][slot] = d[1];
+                stage[5][slot] = d[2];
+                stage_src[slot] = i;
+            }
         }
         __syncthreads();
-        if (t < row_start[j][8]) {
-            // the key of sorted slot t, and where its run of this row goes
-            uint32_t kt = 0;
 #pragma unroll
-            for (uint32_t o = 1; o < 8; o++) kt += t >= row_start[j][o] ? 1u : 0u;
-            const uint32_t k = key_base[kt] + local[(kt * kGenRows + j) * 16] + (t - row_start[j][kt]);
-            out.ox[k] = stage[0][t];
-            out.oy[k] = stage[1][t];
-            out.oz[k] = stage[2][t];
-            out.dx[k] = stage[3][t];
-            out.dy[k] = stage[4][t];
-            out.dz[k] = stage[5][t];
-            src[k] = stage_src[t];
+        for (uint32_t u = t; u < kStage; u += 1024) {
+            if (u < round_start[q][8]) {
+                // the key of sorted slot u, and where its run of this round goes
+                uint32_t ku = 0;
+#pragma unroll
+                for (uint32_t o = 1; o < 8; o++) ku += u >= round_start[q][o] ? 1u : 0u;
+                const uint32_t k = key_base[ku] + local[(ku * kGenRows + q * kStageRows) * 16] + (u - round_start[q][ku]);
+                out.ox[k] = stage[0][u];
+                out.oy[k] = stage[1][u];
+                out.oz[k] = stage[2][u];
+                out.dx[k] = stage[3][u];
+                out.dy[k] = stage[4][u];
+                out.dz[k] = stage[5][u];
+                src[k] = stage_src[u];
+            }
         }
         __syncthreads();
     }
