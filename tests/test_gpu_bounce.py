@@ -289,3 +289,39 @@ def test_direction_order_limits(vrt, cuda):
     n = 8388609
     with pytest.raises(V.VrtError, match="at most 8388608"):
         V.bounce(V.RayBatch.empty(n), V.HitBatch.empty(n, with_pos=True), 4096, 1, order=1)
+
+
+@pytest.mark.parametrize("width", [1, 3, 640, 4097, 65537, 2**31 + 1, 2**32 - 1])
+def test_bounce_generation_any_width(vrt, cuda, width):
+    """The device splits the primary index into (i % width, i / width) with a multiply-high by a
+    per-launch constant: rays and sources equal the host twin's (plain division) at widths
+    from 1 to 2^32 - 1, in both orders."""
+    import voxelrt_amd as V
+    from voxelrt_amd import _lib as L
+
+    n = 20011
+    rng = np.random.default_rng(width % 1000)
+    rec = np.zeros((n, 4), np.int32)
+    hit = rng.random(n) < 0.6
+    codes = rng.integers(0, 6, n)
+    info = np.where(hit, 1 + rng.integers(0, 255, n), 0)
+    rec[:, 3] = np.where(hit, info | (np.where(codes % 2 == 0, 1, 3) << (8 + 2 * (codes // 2))), 0).astype(np.int32)
+    pos = (rng.random((3, n), dtype=np.float32) * 64.0).astype(np.float32)
+    hits = V.HitBatch(torch.zeros(n, device=cuda), torch.from_numpy(rec).to(cuda), torch.from_numpy(pos).to(cuda))
+    prim = V.RayBatch.empty(n, (0, 0, 0))
+    out = np.zeros((6, n), np.float32)
+    hsrc = np.zeros(n, np.uint32)
+    cnt = C.c_uint32()
+    ptrs = (C.c_void_p * 6)(*[out[i].ctypes.data for i in range(6)])
+    L.check(V.lib().vrt_bounce_host(n, rec.ctypes.data, pos.ctypes.data, width, 9, C.byref(ptrs), hsrc.ctypes.data,
+                                    C.byref(cnt)))
+    nb = cnt.value
+    for order in (0, 1):
+        br, src, count = V.bounce(prim, hits, width, 9, order=order)
+        torch.cuda.synchronize()
+        assert int(count.item()) == nb
+        s = src[:nb].cpu().numpy().astype(np.int64)
+        j = np.searchsorted(hsrc[:nb].astype(np.int64), s)
+        assert np.array_equal(hsrc[:nb][j].astype(np.int64), s)
+        assert np.array_equal(br.o[:, :nb].cpu().numpy().view(np.uint32), out[:3, :nb][:, j].view(np.uint32))
+        assert np.array_equal(br.d[:, :nb].cpu().numpy().view(np.uint32), out[3:, :nb][:, j].view(np.uint32))

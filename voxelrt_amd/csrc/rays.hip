@@ -174,7 +174,7 @@ __global__ __launch_bounds__(1024) void bounce_write_rows_kernel(const int4* __r
                                                                  const float* __restrict__ px,
                                                                  const float* __restrict__ py,
                                                                  const float* __restrict__ pz, uint32_t n,
-                                                                 uint32_t width, uint32_t seed,
+                                                                 udiv32 width, uint32_t seed,
                                                                  const uint32_t* __restrict__ block_counts,
                                                                  uint32_t nblocks, vrt_ray_buffers out,
                                                                  uint32_t* __restrict__ src,
@@ -224,8 +224,9 @@ __global__ __launch_bounds__(1024) void bounce_write_rows_kernel(const int4* __r
         uint32_t k = row_base[j * 16 + wave] +
                      __builtin_amdgcn_mbcnt_hi((uint32_t)(ball[j] >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)ball[j], 0u));
         float p[3] = {px[i], py[i], pz[i]};
-        float o[3], d[3];
-        bounce_ray((uint32_t)r[j].w, p, i, width, seed, o, d);
+        float o[3], d[3], u1, u2;
+        bounce_samples(i, width, seed, u1, u2);
+        bounce_ray_samples((uint32_t)r[j].w, p, u1, u2, o, d);
         out.ox[k] = o[0];
         out.oy[k] = o[1];
         out.oz[k] = o[2];
@@ -244,12 +245,10 @@ __global__ __launch_bounds__(1024) void bounce_write_rows_kernel(const int4* __r
 // s = (sin(2 pi u2) sy, 2 u1 - 1, cos(2 pi u2) sy) with sy >= 0.  It is d's octant except
 // for rays with a component at a sign boundary; both passes compute it with this function,
 // so the placement never depends on rounding.  8: no hit, no bounce ray.  u1, u2: the samples.
-__device__ __forceinline__ uint32_t bounce_key(uint32_t info, uint32_t i, uint32_t width, uint32_t seed, float& u1,
-                                               float& u2) {
+__device__ __forceinline__ uint32_t bounce_key(uint32_t info, uint32_t i, const udiv32& width, uint32_t seed,
+                                               float& u1, float& u2) {
     if ((info & 0xFFu) == 0) return 8u;
-    uint32_t st = pcg_init(i % width, i / width, seed);
-    u1 = pcg_unit(st);
-    u2 = pcg_unit(st);
+    bounce_samples(i, width, seed, u1, u2);
     const uint32_t cx = (info >> 8) & 3u, cy = (info >> 10) & 3u, cz = (info >> 12) & 3u;  // 1: +, 3: -
     const bool bx = cx ? cx == 1u : u2 < 0.5f;
     const bool by = cy ? cy == 1u : u1 >= 0.5f;
@@ -271,7 +270,7 @@ __device__ __forceinline__ uint64_t same_key_lanes(uint32_t key, uint32_t pops[8
 
 // Per block of kGenBlock primaries, the hits of each key: counts[key * nblocks + block].
 __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void bounce_count_keys_kernel(const int4* __restrict__ rec, uint32_t n,
-                                                                 uint32_t width, uint32_t seed, uint32_t nblocks,
+                                                                 udiv32 width, uint32_t seed, uint32_t nblocks,
                                                                  uint32_t* __restrict__ counts) {
     __shared__ uint32_t wave_counts[16][8];
     const uint32_t t = threadIdx.x, b = blockIdx.x;
@@ -306,7 +305,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void 
                                                                  const float* __restrict__ px,
                                                                  const float* __restrict__ py,
                                                                  const float* __restrict__ pz, uint32_t n,
-                                                                 uint32_t width, uint32_t seed,
+                                                                 udiv32 width, uint32_t seed,
                                                                  const uint32_t* __restrict__ counts,
                                                                  uint32_t nblocks, vrt_ray_buffers out,
                                                                  uint32_t* __restrict__ src,
@@ -509,16 +508,17 @@ static int bounce_generate(const vrt_rays* primary, const vrt_hits* hits, uint32
     const int4* rec = (const int4*)hits->rec;
     const uint32_t nrows = (n + kGenBlock - 1) / kGenBlock;
     if (order == VRT_BOUNCE_DIRECTION_ORDER) {  // nrows <= kGenMaxBlocks; blocks holds 8 * nrows counts
-        bounce_count_keys_kernel<<<nrows, 1024, 0, s>>>(rec, n, width, seed, nrows, blocks);
+        const udiv32 wd = make_udiv32(width);
+        bounce_count_keys_kernel<<<nrows, 1024, 0, s>>>(rec, n, wd, seed, nrows, blocks);
         bounce_write_keys_kernel<<<nrows, 1024, 0, s>>>(rec, hits->pos, hits->pos + n, hits->pos + 2 * (size_t)n, n,
-                                                        width, seed, blocks, nrows, *out, src, count);
+                                                        wd, seed, blocks, nrows, *out, src, count);
         VRT_HIP(hipGetLastError());
         return VRT_OK;
     }
     if (nrows <= kGenMaxBlocks) {
         bounce_count_rows_kernel<<<nrows, 1024, 0, s>>>(rec, n, blocks);
         bounce_write_rows_kernel<<<nrows, 1024, 0, s>>>(rec, hits->pos, hits->pos + n, hits->pos + 2 * (size_t)n, n,
-                                                        width, seed, blocks, nrows, *out, src, count);
+                                                        make_udiv32(width), seed, blocks, nrows, *out, src, count);
         VRT_HIP(hipGetLastError());
         return VRT_OK;
     }

@@ -123,6 +123,37 @@ __host__ __device__ inline void cosine_dir(const float N[3], uint32_t& st, float
 // Diffuse bounce of one hit (CpuRenderer.cpp:389-392, VoxelRender.comp:63-71):
 // origin = pos + N * 0.01, direction = normalize(N + s), s uniform on the unit
 // sphere from two PCG samples; N + s = 0 falls back to N.
+// n / d for a divisor fixed per launch (d >= 1): a multiply-high and two shifts (Granlund and
+// Montgomery, "Division by invariant integers using multiplication", PLDI 1994: the round-up
+// multiplier m = floor(2^32 (2^l - d) / d) + 1, l = ceil(log2 d), with its add-and-shift step),
+// exact for every 32-bit n.
+struct udiv32 {
+    uint32_t d, m, s1, s2;
+};
+inline udiv32 make_udiv32(uint32_t d) {
+    uint32_t l = 0;
+    while (l < 32 && (1ull << l) < d) l++;
+    udiv32 v;
+    v.d = d;
+    v.m = (uint32_t)((((1ull << l) - d) << 32) / d + 1);
+    v.s1 = l < 1 ? l : 1;
+    v.s2 = l > 1 ? l - 1 : 0;
+    return v;
+}
+__host__ __device__ inline uint32_t udiv(const udiv32& v, uint32_t n) {
+    const uint32_t t = (uint32_t)(((uint64_t)v.m * n) >> 32);
+    return (t + ((n - t) >> v.s1)) >> v.s2;
+}
+
+// Hit i's two PCG samples: the first two of pcg_init(i % width, i / width, seed), with the
+// division by the width as udiv.
+__host__ __device__ inline void bounce_samples(uint32_t i, const udiv32& width, uint32_t seed, float& u1, float& u2) {
+    const uint32_t py = udiv(width, i), px = i - py * width.d;
+    uint32_t st = pcg_init(px, py, seed);
+    u1 = pcg_unit(st);
+    u2 = pcg_unit(st);
+}
+
 // The same from hit i's two PCG samples (u1, u2: the first two of pcg_init(i % width, i / width,
 // seed)), for callers that already drew them.
 __host__ __device__ inline void bounce_ray_samples(uint32_t info, const float pos[3], float u1, float u2, float o[3],
