@@ -16,7 +16,9 @@
  * is asynchronous on the given stream (a hipStream_t, NULL = the null stream);
  * buffers passed to casts are device pointers owned by the caller.  A call with
  * nothing to do (no rays, views or pixels) checks its options, launches nothing
- * and returns VRT_OK; its buffer pointers may then be NULL.
+ * and returns VRT_OK; its buffer pointers may then be NULL.  A world lives on the
+ * device current at vrt_world_create; calls that take the world (or a batch plan
+ * of it) need that device current, and return VRT_ERR_STATE otherwise.
  *
  * Reference interfaces replaced (paths relative to the VoxelRT source tree):
  *   vrt_world_*            GpuVoxelStorage / FlatVoxelStorage SyncBuffers

@@ -247,6 +247,12 @@ int vrt::check_world(const vrt_world* w, const char* entry) {
         return fail(VRT_ERR_STATE, std::string(entry) +
                                        ": a vrt_world_sync failed after its device work began, so the world's "
                                        "device tables are inconsistent; destroy the world");
+    // the world's buffers, and the state kept per (device, stream), live on the device that
+    // was current at vrt_world_create: a call with another one current would allocate on it
+    int cur = -1;
+    if (w && hipGetDevice(&cur) == hipSuccess && cur != w->device)
+        return fail(VRT_ERR_STATE, std::string(entry) + ": device " + std::to_string(cur) +
+                                       " is current, the world lives on device " + std::to_string(w->device));
     return VRT_OK;
 }
 
