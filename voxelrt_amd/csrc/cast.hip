@@ -457,9 +457,10 @@ static int survivor_state(int dev, hipStream_t s, uint32_t n, SurvivorState* q) 
 
 // ---------------------------------------------------------------- launch
 
-// First-pass steps of the compact schedule when the caller gives 0: the best of 8-64 on
-// 2 M incoherent rays in the 2048^3 world of 20 000 spheres (0.367 against 0.416 ms for
-// the grid); 64 is best in the 587-sphere world (0.472 against 0.488 ms).
+// First-pass steps of the compact schedule when the caller gives 0: within 1% of the best
+// of 24-64 on 2 M incoherent rays in the 2048^3 world of 20 000 spheres (0.317 ms, 0.314 at
+// 24, against 0.341 ms for the grid); 64 is best in the 587-sphere world (0.432 against
+// 0.448 ms).
 constexpr uint32_t kDefaultPassCap = 32;
 
 // Option checks that do not depend on the ray count, so a call is rejected or
