@@ -286,7 +286,16 @@ __global__ __launch_bounds__(256) void bwd_place_kernel(uint32_t n, const uint32
 // min(nvox, slot * 512 + 512)) of the gradient: every entry is written, zeros where no pixel hit.
 // On the 512^3 terrain's 97 k slots this moves 598 MB of gradient and ~145 MB of gathered pixel
 // data in 0.133 ms (5.6 TB/s); four slots per workgroup (0.145 ms) and a grid stride over 8192
-// long-lived waves (0.156 ms) measured slower.
+// long-lived waves (0.156 ms) measured slower.  The gradient rows are written with non-temporal
+// stores (nothing in the pass reads them back): 2% off the backward in alternating runs.
+__device__ __forceinline__ void store_row4_nt(float4* dst, float4 x) {
+    float* q = (float*)dst;
+    __builtin_nontemporal_store(x.x, q + 0);
+    __builtin_nontemporal_store(x.y, q + 1);
+    __builtin_nontemporal_store(x.z, q + 2);
+    __builtin_nontemporal_store(x.w, q + 3);
+}
+
 __global__ __launch_bounds__(64) void bwd_reduce_kernel(uint64_t nvox, const float* __restrict__ grad_image,
                                                         const uint32_t* __restrict__ saved_voxel,
                                                         const float* __restrict__ saved_weight,
@@ -304,7 +313,7 @@ __global__ __launch_bounds__(64) void bwd_reduce_kernel(uint64_t nvox, const flo
     if (cnt == 0) {
         if (rows == 512) {
             float4* o4 = (float4*)out;
-            for (uint32_t k = lane; k < 384; k += 64) o4[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            for (uint32_t k = lane; k < 384; k += 64) store_row4_nt(&o4[k], make_float4(0.0f, 0.0f, 0.0f, 0.0f));
         } else {
             for (uint32_t k = lane; k < rows * 3; k += 64) out[k] = 0.0f;
         }
@@ -325,7 +334,7 @@ __global__ __launch_bounds__(64) void bwd_reduce_kernel(uint64_t nvox, const flo
     if (rows == 512) {
         float4* o4 = (float4*)out;
         const float4* a4 = (const float4*)acc;
-        for (uint32_t k = lane; k < 384; k += 64) o4[k] = a4[k];
+        for (uint32_t k = lane; k < 384; k += 64) store_row4_nt(&o4[k], a4[k]);
     } else {
         for (uint32_t k = lane; k < rows * 3; k += 64) out[k] = acc[k];
     }
