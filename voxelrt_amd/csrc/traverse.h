@@ -38,7 +38,8 @@
 // reads an occupancy mask.  The loop keeps no hit flag: ray_trace tells the exits
 // apart after the loop (by the step count, or by one lookup at any origin).  Each
 // exit and flag costs lane-mask bookkeeping on the scalar unit, which one CU's four
-// SIMDs share: the one-exit step issues 16 scalar ALU instructions and 4 branches.
+// SIMDs share: the one-exit step issues 18 scalar ALU instructions and 5 branches (one of
+// them skips the sector load when no lane needs it).
 #pragma once
 
 #include <hip/hip_runtime.h>
